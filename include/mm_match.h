@@ -1,0 +1,175 @@
+/*
+ * mm_match.h — C ABI of the MI355X (gfx950) fine-grained greedy matcher.
+ *
+ * Plain pointers and sizes only; no C++ or torch types cross this boundary.
+ * Every entry point returns MM_OK (0) or a positive MM_ERR_* code and leaves a
+ * message for mm_last_error().  The C++ classes in
+ * multicore-matching_amd/csrc/matchgpu.h (GraphMatchingGPU*) sit on top of it.
+ *
+ * Reference interfaces replaced (BasFaggingerAuer/Multicore-Matching):
+ *   mm_graph_create   <- GraphMatchingGPU ctor, device upload of the Graph CSR
+ *                        (src/matchgpu.cu:39-66, weights 97-141; src/matchgpu.h:29-42)
+ *   mm_graph_destroy  <- GraphMatchingGPU* dtors (src/matchgpu.cu:68-73, 117-121, 143-147)
+ *   mm_match          <- GraphMatchingGPU{Random,RandomMaximal,Weighted,WeightedMaximal}
+ *                        ::performMatching (src/matchgpu.cu:453-851) and its kernels
+ *                        gSelect/gaSelect/grRequest/grRespond/gwRequest/gwRespond/gMatch
+ *                        (src/matchgpu.cu:184-451)
+ *   mm_batch_*        <- no counterpart: many independent graphs per launch
+ *   mm_gen_*          <- no counterpart: seeded synthetic workload generators
+ */
+#ifndef MM_MATCH_H
+#define MM_MATCH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_OK 0
+#define MM_ERR_INVALID 1   /* bad argument or malformed CSR */
+#define MM_ERR_NOMEM 2     /* device or host allocation failed */
+#define MM_ERR_HIP 3       /* HIP runtime / kernel error */
+#define MM_ERR_SEEDS 4     /* fewer seeds than rounds to run */
+
+/* Matching kinds: src/matchgpu.cu:286-357 (random), 360-451 (weighted). */
+#define MM_RANDOM 0
+#define MM_WEIGHTED 1
+
+/* Round counts of the reference loops (src/matchcpu.h:24-25). */
+#define MM_NR_MATCH_ROUNDS 20       /* fixed variants, types 6 and 8 */
+#define MM_NR_MAX_MATCH_ROUNDS 255  /* maximal variants: while (keep && ++count < 256) */
+#define MM_MAX_ROUNDS 255
+/* Default select barrier, p = 0.534059 (bin/match.cpp:182-183). */
+#define MM_DEFAULT_BARRIER 0x88B81733u
+
+/* Respond strategy.  PULL scans every red vertex's row exactly as the
+ * reference does and is exact for any CSR.  PUSH lets each blue vertex post
+ * its proposal with one 64-bit atomic max; it returns the same match array
+ * when the CSR is structurally symmetric, every row is strictly ascending and
+ * w(u,v) == w(v,u) — mm_graph_create checks this and AUTO picks PUSH only
+ * then. */
+#define MM_RESPOND_AUTO 0
+#define MM_RESPOND_PULL 1
+#define MM_RESPOND_PUSH 2
+
+typedef struct mm_graph mm_graph;
+
+typedef struct mm_opts {
+    int32_t kind;             /* MM_RANDOM or MM_WEIGHTED */
+    int32_t maximal;          /* 0: exactly max_rounds rounds; 1: stop after the first round whose
+                                 select saw no active vertex, at most max_rounds rounds */
+    int32_t max_rounds;       /* 1..MM_MAX_ROUNDS; 0 = 20 (fixed) or 255 (maximal) */
+    uint32_t barrier;         /* blue iff hash < barrier */
+    const uint32_t *seeds;    /* host array, one seed per round, >= rounds that can run */
+    int32_t n_seeds;
+    int32_t block_size;       /* threads per block, multiple of 64 in [64, 1024]; 0 = 256 */
+    int32_t lanes_per_vertex; /* 1,2,4,...,64 lanes scan one row; 0 = chosen from the mean degree */
+    int32_t hub_degree;       /* rows longer than this are scanned by a whole block; 0 = 256 */
+    int32_t respond;          /* MM_RESPOND_* */
+    int32_t profile;          /* 1: time every kernel with HIP events (slower; fills kernel_ms) */
+} mm_opts;
+
+#define MM_KERNEL_CLASSES 5   /* 0 init-select, 1 request, 2 respond, 3 match-select, 4 clear */
+
+typedef struct mm_stats {
+    int32_t rounds;                    /* rounds executed = seeds consumed */
+    int32_t respond_used;              /* MM_RESPOND_PULL or MM_RESPOND_PUSH */
+    int32_t lanes_used;                /* lanes per vertex that ran */
+    int32_t reserved;
+    float match_ms;                    /* device time from before the first round to after the last
+                                          (the reference's t1 -> t2, src/matchgpu.cu:500-527) */
+    float kernel_ms[MM_KERNEL_CLASSES];/* profile == 1: summed event time per kernel class */
+    int32_t kernel_launches[MM_KERNEL_CLASSES];
+    int64_t active[MM_MAX_ROUNDS + 1]; /* [r] vertices with match < 2 at the select of round r+1 */
+    int64_t blue[MM_MAX_ROUNDS + 1];   /* [r] of those, coloured blue */
+    int64_t matched[MM_MAX_ROUNDS + 1];/* [r] matched vertices after round r+1 (cumulative) */
+    int64_t active_nnz[MM_MAX_ROUNDS + 1]; /* [r] sum of degrees of the active vertices (profile == 1) */
+    int64_t blue_nnz[MM_MAX_ROUNDS + 1];   /* [r] sum of degrees of the blue vertices (profile == 1) */
+} mm_stats;
+
+typedef struct mm_graph_info {
+    int32_t n;
+    int32_t weighted;
+    int64_t nnz;
+    int32_t max_degree;
+    int32_t push_ok;          /* symmetric, strictly ascending rows, symmetric weights */
+    int64_t device_bytes;     /* device memory held by the handle */
+} mm_graph_info;
+
+/* Last error message of the calling thread ("" when none). */
+const char *mm_last_error(void);
+/* Number of visible HIP devices (0 when there is no GPU or no runtime). */
+int32_t mm_device_count(void);
+
+/* Upload a CSR graph in the reference layout (src/graph.h:41-60):
+ *   ranges      int32[2n]: ranges[2i] = begin, ranges[2i+1] = end of row i (the Graph's int2)
+ *   neighbours  int32[nnz]
+ *   weights     float[nnz] or NULL (unit weights; MM_WEIGHTED then sees all ties)
+ * The device copy is owned by the handle.  device = HIP device ordinal. */
+int32_t mm_graph_create(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges,
+                        const int32_t *neighbours, const float *weights, mm_graph **out);
+int32_t mm_graph_destroy(mm_graph *g);
+int32_t mm_graph_get_info(const mm_graph *g, mm_graph_info *info);
+
+/* Run one matching.  match_out (host, n entries) receives the final state array
+ * in the reference encoding (0 blue, 1 red, 2 dead, >=4 matched: 4 + min(i, partner));
+ * pass NULL to leave the result on the device (mm_graph_copy_match fetches it).
+ * stats may be NULL. */
+int32_t mm_match(mm_graph *g, const mm_opts *opts, int32_t *match_out, mm_stats *stats);
+int32_t mm_graph_copy_match(mm_graph *g, int32_t *match_out);
+
+/* Batched matching: many independent graphs concatenated into one CSR.
+ *   vertex_offsets int32[n_graphs+1], edge_offsets int64[n_graphs+1]
+ *   ranges int32[2 * total_n] with positions LOCAL to each graph's edge block,
+ *   neighbours int32[total_nnz] with ids LOCAL to each graph, weights or NULL.
+ * One launch sequence matches every graph; graph g's result equals mm_match on
+ * graph g alone with the same seeds (hash input, sentinels and match values
+ * use graph-local ids). */
+typedef struct mm_batch mm_batch;
+int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_offsets,
+                        const int64_t *edge_offsets, const int32_t *ranges,
+                        const int32_t *neighbours, const float *weights, mm_batch **out);
+int32_t mm_batch_destroy(mm_batch *b);
+/* rounds_out int32[n_graphs] (may be NULL): rounds graph g executed alone. */
+int32_t mm_batch_match(mm_batch *b, const mm_opts *opts, int32_t *match_out,
+                       int32_t *rounds_out, mm_stats *stats);
+
+/* Result metrics of a match array (host; src/matchcpu.cu:64-97 getWeight and
+ * 127-246 testMatching).  mm_get_weight: size = matched VERTICES, weight summed
+ * in double over both endpoints and halved.  mm_test_matching fills
+ * counts[4] = {matched to themselves, matched to more than one vertex, not
+ * matched to a neighbour, still matchable} and returns MM_OK when the first
+ * three are zero (MM_ERR_INVALID otherwise). */
+int32_t mm_get_weight(int32_t n, const int32_t *ranges, const int32_t *neighbours, const float *weights,
+                      const int32_t *match, double *weight_out, int64_t *size_out);
+int32_t mm_test_matching(int32_t n, const int32_t *ranges, const int32_t *neighbours, const int32_t *match,
+                         int64_t *counts);
+
+/* Seeded synthetic CSR generators (host).  Arrays are malloc'd by the library
+ * and released with mm_csr_free.  Rows are strictly ascending; weights are
+ * w = (k + 1) / 2^24 with k a hash of (seed, min(u,v), max(u,v)), mirrored. */
+typedef struct mm_csr {
+    int32_t n;
+    int64_t nnz;
+    int32_t *ranges;      /* int32[2n] */
+    int32_t *neighbours;  /* int32[nnz] */
+    float *weights;       /* float[nnz] or NULL */
+} mm_csr;
+/* Random geometric graph: n points uniform in [0,1)^2, edge iff distance < r,
+ * r = radius_scale * sqrt(ln n / n) (radius_scale 0.55 for the standard workload). */
+int32_t mm_gen_rgg(int32_t n, double radius_scale, uint64_t seed, int32_t weighted, mm_csr *out);
+/* R-MAT: 2^scale vertices, edge_factor * 2^scale samples, quadrant
+ * probabilities a, b, c (d = 1 - a - b - c); symmetrised, deduplicated,
+ * self-loops removed. */
+int32_t mm_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                    uint64_t seed, int32_t weighted, mm_csr *out);
+/* rows x cols 2-D grid, 4-neighbour stencil, vertex id = r * cols + c. */
+int32_t mm_gen_grid(int32_t rows, int32_t cols, uint64_t seed, int32_t weighted, mm_csr *out);
+void mm_csr_free(mm_csr *csr);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MM_MATCH_H */

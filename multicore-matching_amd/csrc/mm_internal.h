@@ -1,0 +1,21 @@
+// Internal helpers shared by the host sources of the matcher library.
+#ifndef MM_INTERNAL_H
+#define MM_INTERNAL_H
+
+#include <cstdint>
+#include <functional>
+
+namespace mm {
+
+// Stores msg as the calling thread's last error and returns code.
+int32_t set_error(int32_t code, const char *msg);
+int32_t set_errorf(int32_t code, const char *fmt, ...);
+void clear_error();
+
+// Static partition of [0, total) over host worker threads (MM_GEN_THREADS or
+// all hardware threads); body(begin, end, worker).
+void parallel_for(int64_t total, const std::function<void(int64_t, int64_t, unsigned)> &body);
+
+}  // namespace mm
+
+#endif

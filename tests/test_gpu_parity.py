@@ -1,0 +1,200 @@
+"""GPU parity: the HIP matcher (through the C ABI) against the golden fixtures
+and the CPU oracle.  Integer/index work, so the bar is bit-exact equality of
+the whole match array and of the round count.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import mmamd
+import oracle
+from mmamd import (CSR, MM_RANDOM, MM_RESPOND_PULL, MM_RESPOND_PUSH, MM_WEIGHTED, Batch, Graph, reference_seeds)
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SEEDS = reference_seeds(255)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def case_csr(c):
+    w = np.array(c["weight_bits"], dtype=np.uint32).view(np.float32)
+    return CSR.from_offsets(c["offsets"], np.array(c["neighbours"], np.int32), w)
+
+
+def check_against_oracle(csr, g, weighted, maximal, **kw):
+    exp, R, act, mat = oracle.match(csr, weighted, maximal, SEEDS, nthreads=0)
+    res = g.match(kind=MM_WEIGHTED if weighted else MM_RANDOM, maximal=maximal, seeds=SEEDS, **kw)
+    assert res.rounds == R
+    mism = np.flatnonzero(res.match != exp)
+    assert mism.size == 0, f"{mism.size} mismatches, first at {mism[:5]}"
+    np.testing.assert_array_equal(res.active, act)
+    np.testing.assert_array_equal(res.matched, mat)
+    return res
+
+
+def test_device_present():
+    assert mmamd.lib().mm_device_count() >= 1
+
+
+def test_golden_tiny_cases_all_types_and_tiles():
+    with open(os.path.join(HERE, "golden", "tiny_cases.json")) as f:
+        data = json.load(f)
+    seeds = np.array(data["seeds"], np.uint32)
+    for idx, c in enumerate(data["cases"]):
+        say("case", idx, c["spec"], "weighted", c["weighted"], "maximal", c["maximal"])
+        csr = case_csr(c)
+        g = Graph(csr)
+        modes = [MM_RESPOND_PULL] + ([MM_RESPOND_PUSH] if g.info()["push_ok"] else [])
+        for respond in modes:
+            for lanes, block, hub in ((1, 64, 2), (4, 256, 256), (16, 256, 3), (64, 128, 1)):
+                res = g.match(kind=MM_WEIGHTED if c["weighted"] else MM_RANDOM, maximal=c["maximal"], seeds=seeds,
+                              max_rounds=c["max_rounds"], barrier=c["barrier"], lanes=lanes, block_size=block,
+                              hub_degree=hub, respond=respond)
+                assert res.rounds == c["rounds"], (idx, respond, lanes)
+                assert res.match.tolist() == c["match"], (idx, respond, lanes, block, hub)
+        g.close()
+
+
+@pytest.mark.parametrize("maximal", [False, True])
+def test_rgg16_random_bitexact(maximal):
+    """RGG n=2^16, unit weights, random matching (types 6/7), one seed per round."""
+    say("rgg16 random maximal", maximal)
+    csr = mmamd.gen_rgg(1 << 16, seed=12345, weighted=False)
+    g = Graph(csr)
+    res = check_against_oracle(csr, g, False, maximal)
+    assert oracle.test_matching(csr, res.match)[0]
+
+
+@pytest.mark.parametrize("maximal", [False, True])
+def test_rgg18_weighted_bitexact(maximal):
+    say("rgg18 weighted maximal", maximal)
+    csr = mmamd.gen_rgg(1 << 18, seed=777, weighted=True)
+    g = Graph(csr)
+    assert g.info()["push_ok"] == 1
+    for respond in (MM_RESPOND_PULL, MM_RESPOND_PUSH):
+        res = check_against_oracle(csr, g, True, maximal, respond=respond)
+        assert res.respond == respond
+    assert oracle.test_matching(csr, res.match)[0]
+
+
+def test_unit_weights_weighted_equals_random_on_gpu():
+    say("unit weights identity")
+    csr = mmamd.gen_rgg(1 << 15, seed=5, weighted=False)
+    g = Graph(csr)
+    a = g.match(kind=MM_RANDOM, maximal=True, seeds=SEEDS)
+    b = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS)
+    assert a.rounds == b.rounds and np.array_equal(a.match, b.match)
+
+
+def shuffled(csr, seed):
+    """The reference's random_shuffle (src/graph.cu:513-548): rows move with
+    their vertex, neighbour ids are relabelled in place, so rows end up
+    unsorted."""
+    rng = np.random.default_rng(seed)
+    n = csr.n
+    perm = rng.permutation(n)
+    inv = np.empty(n, np.int64)
+    inv[perm] = np.arange(n)
+    ranges = csr.ranges[perm]
+    nb = inv[csr.neighbours].astype(np.int32)
+    return CSR(np.ascontiguousarray(ranges, dtype=np.int32), nb, csr.weights)
+
+
+def test_shuffled_rows_use_exact_pull_path():
+    say("shuffled rgg")
+    csr = shuffled(mmamd.gen_rgg(1 << 16, seed=31, weighted=True), 4)
+    g = Graph(csr)
+    assert g.info()["push_ok"] == 0
+    for weighted in (False, True):
+        res = check_against_oracle(csr, g, weighted, True)
+        assert res.respond == MM_RESPOND_PULL
+
+
+def test_tile_choices_do_not_change_results():
+    say("tile determinism")
+    csr = mmamd.gen_rgg(1 << 15, seed=8, weighted=True)
+    g = Graph(csr)
+    ref = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS)
+    for lanes in (1, 2, 4, 8, 16, 32, 64):
+        for block in (64, 256, 1024):
+            for hub in (4, 12, 256):
+                res = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS, lanes=lanes, block_size=block,
+                              hub_degree=hub)
+                assert res.rounds == ref.rounds and np.array_equal(res.match, ref.match), (lanes, block, hub)
+
+
+@pytest.mark.parametrize("hub", [16, 256])
+def test_rmat_hub_path_bitexact(hub):
+    say("rmat scale 15 hub", hub)
+    csr = mmamd.gen_rmat(15, 16, seed=99, weighted=True)
+    g = Graph(csr)
+    assert g.info()["max_degree"] > 256
+    for weighted in (False, True):
+        for respond in (MM_RESPOND_PULL, MM_RESPOND_PUSH):
+            check_against_oracle(csr, g, weighted, True, hub_degree=hub, respond=respond)
+
+
+def test_batch_equals_single_graph_path():
+    say("batch of mixed graphs")
+    rng = np.random.default_rng(3)
+    graphs = []
+    for k in range(40):
+        n = int(2 ** rng.uniform(4, 11))
+        if k % 2:
+            side = max(1, int(np.sqrt(n)))
+            graphs.append(mmamd.gen_grid(side, max(1, n // side), seed=k, weighted=True))
+        else:
+            graphs.append(mmamd.gen_rgg(n, seed=k, weighted=True))
+    graphs.append(mmamd.gen_rgg(0, seed=1))
+    graphs.append(mmamd.gen_grid(1, 1, seed=1))
+    b = Batch(graphs)
+    for maximal in (False, True):
+        res, per, rounds = b.match(kind=MM_WEIGHTED, maximal=maximal, seeds=SEEDS)
+        for k, csr in enumerate(graphs):
+            exp, R, _, _ = oracle.match(csr, True, maximal, SEEDS)
+            assert per[k].tolist() == exp.tolist(), k
+            assert rounds[k] == R, (k, rounds[k], R)
+        assert res.rounds == max(rounds)
+
+
+def test_empty_graph_and_bad_arguments():
+    say("edge cases")
+    g = Graph(CSR.from_offsets([0], np.zeros(0, np.int32)))
+    res = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS)
+    assert res.rounds == 1 and res.match.size == 0
+    res = g.match(kind=MM_WEIGHTED, maximal=False, seeds=SEEDS)
+    assert res.rounds == 20
+    csr = mmamd.gen_rgg(1000, seed=2)
+    g = Graph(csr)
+    with pytest.raises(mmamd.MatchError):
+        g.match(maximal=False, seeds=SEEDS[:5])
+    with pytest.raises(mmamd.MatchError):
+        g.match(maximal=True, seeds=SEEDS[:2])  # cannot converge in 2 rounds
+    with pytest.raises(mmamd.MatchError):
+        g.match(seeds=SEEDS, lanes=3)
+    with pytest.raises(mmamd.MatchError):
+        g.match(seeds=SEEDS, block_size=100)
+    # a failed call leaves the handle usable
+    exp, R, _, _ = oracle.match(csr, True, True, SEEDS)
+    assert g.match(seeds=SEEDS).match.tolist() == exp.tolist()
+
+
+@pytest.mark.slow
+def test_rgg22_weighted_full_size_bitexact():
+    """RGG n=2^22 weighted (the benchmark workload): bit-exact and valid."""
+    say("rgg22 weighted full size")
+    csr = mmamd.gen_rgg(1 << 22, seed=12345, weighted=True)
+    g = Graph(csr)
+    res = check_against_oracle(csr, g, True, True)
+    ok, under, multi, invalid, matchable = oracle.test_matching(csr, res.match)
+    assert ok and matchable == 0
+    w_gpu, size = oracle.get_weight(csr, res.match)
+    assert size == res.matched[-1]
