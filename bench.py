@@ -93,7 +93,20 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--respond", type=int, default=mmamd.MM_RESPOND_AUTO)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--keep-order", action="store_true", help="no internal breadth-first renumbering")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="run only a plain device copy of this many MiB (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        x = torch.ones(args.copy_probe << 18, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe << 20, "launches": 3}), flush=True)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; multi-GPU sharding is a design note, "
               "not a measured path. Run with --gpus 1.", file=sys.stderr)
@@ -106,7 +119,9 @@ def main():
     t0 = time.perf_counter()
     csr = mmamd.gen_rgg(n, seed=args.seed, weighted=True)
     log(f"generated RGG n={n} nnz={csr.nnz} m={csr.n_edges} in {time.perf_counter() - t0:.2f} s")
-    g = mmamd.Graph(csr)
+    t0 = time.perf_counter()
+    g = mmamd.Graph(csr, keep_order=args.keep_order)
+    log(f"graph upload (incl. renumbering) {time.perf_counter() - t0:.2f} s")
     seeds = mmamd.reference_seeds(mmamd.MM_MAX_ROUNDS)
     tiles = dict(lanes=args.lanes, block_size=args.block, respond=args.respond)
 
@@ -137,7 +152,7 @@ def main():
                    **tiles)
     push = prof.respond == mmamd.MM_RESPOND_PUSH
     kb = kernel_bytes(prof, r99, push)
-    kms = {k: v for k, v in prof.kernel_ms.items() if k != "clear"}
+    kms = {k: v for k, v in prof.kernel_ms.items() if k != "counters"}
     dom = max(kms, key=kms.get)
     launches = prof.kernel_launches[dom]
     achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9

@@ -64,18 +64,21 @@ typedef struct mm_opts {
     const uint32_t *seeds;    /* host array, one seed per round, >= rounds that can run */
     int32_t n_seeds;
     int32_t block_size;       /* threads per block, multiple of 64 in [64, 1024]; 0 = 256 */
-    int32_t lanes_per_vertex; /* 1,2,4,...,64 lanes scan one row; 0 = chosen from the mean degree */
+    int32_t lanes_per_vertex; /* 0: entry-parallel tiles (rows staged in LDS, one row entry per thread);
+                                 1,2,4,...,64: that many lanes scan one row */
     int32_t hub_degree;       /* rows longer than this are scanned by a whole block; 0 = 256 */
     int32_t respond;          /* MM_RESPOND_* */
-    int32_t profile;          /* 1: time every kernel with HIP events (slower; fills kernel_ms) */
+    int32_t profile;          /* 1: time every kernel with HIP events (slower; fills kernel_ms and *_nnz) */
+    int32_t graph_mode;       /* 0: capture each launch sequence once as a hipGraph and replay it;
+                                 1: plain stream launches (profile == 1 implies 1) */
 } mm_opts;
 
-#define MM_KERNEL_CLASSES 5   /* 0 init-select, 1 request, 2 respond, 3 match-select, 4 clear */
+#define MM_KERNEL_CLASSES 5   /* 0 init-select, 1 request, 2 respond, 3 match-select, 4 counters */
 
 typedef struct mm_stats {
     int32_t rounds;                    /* rounds executed = seeds consumed */
     int32_t respond_used;              /* MM_RESPOND_PULL or MM_RESPOND_PUSH */
-    int32_t lanes_used;                /* lanes per vertex that ran */
+    int32_t lanes_used;                /* lanes per row that ran (0: entry-parallel tiles) */
     int32_t reserved;
     float match_ms;                    /* device time from before the first round to after the last
                                           (the reference's t1 -> t2, src/matchgpu.cu:500-527) */
@@ -109,6 +112,13 @@ int32_t mm_device_count(void);
  * The device copy is owned by the handle.  device = HIP device ordinal. */
 int32_t mm_graph_create(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges,
                         const int32_t *neighbours, const float *weights, mm_graph **out);
+/* As mm_graph_create, with flags.  By default the handle renumbers the vertices
+ * internally in breadth-first order, so that neighbours sit close together in
+ * memory; results, hash inputs and tie-breaks still use the caller's ids.
+ * MM_CREATE_KEEP_ORDER keeps the caller's numbering on the device. */
+#define MM_CREATE_KEEP_ORDER 1u
+int32_t mm_graph_create_ex(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges,
+                           const int32_t *neighbours, const float *weights, uint32_t flags, mm_graph **out);
 int32_t mm_graph_destroy(mm_graph *g);
 int32_t mm_graph_get_info(const mm_graph *g, mm_graph_info *info);
 

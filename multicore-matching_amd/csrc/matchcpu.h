@@ -1,0 +1,42 @@
+// Matcher base class of the drop-in interface: the reference's
+// mtc::GraphMatching (src/matchcpu.h:24-48) with the same static helpers and
+// state encoding.  performMatching takes hipEvent_t where the reference takes
+// cudaEvent_t; the caller records nothing itself, the matcher records t1
+// after set-up and t2 after the last round (src/matchgpu.cu:500-527).
+#ifndef MM_DROPIN_MATCHCPU_H
+#define MM_DROPIN_MATCHCPU_H
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "graph.h"
+
+#define NR_MATCH_ROUNDS 20
+#define NR_MAX_MATCH_ROUNDS 256
+
+namespace mtc
+{
+
+class GraphMatching
+{
+    public:
+        GraphMatching(const Graph &);
+        virtual ~GraphMatching();
+
+        static void getWeight(double &, long &, const std::vector<int> &, const Graph &);
+        static bool testMatching(const std::vector<int> &, const Graph &);
+
+        static inline int matchVal(const int &i, const int &j) {return 4 + (i < j ? i : j);};
+        static inline bool isMatched(const int &m) {return m >= 4;};
+
+        std::vector<int> initialMatching() const;
+        virtual void performMatching(std::vector<int> &, hipEvent_t &, hipEvent_t &) const = 0;
+
+    protected:
+        const Graph &graph;
+};
+
+}  // namespace mtc
+
+#endif

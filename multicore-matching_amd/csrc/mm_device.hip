@@ -55,6 +55,8 @@ struct Dev {
     int32_t hub_deg;
     uint32_t barrier;
     int32_t count_nnz;
+    int32_t nb;           // buckets = blocks of the per-bucket kernels
+    int32_t bcap;         // capacity of one bucket; bucket b owns ids [b*bcap, (b+1)*bcap) in round 1
     const int2 *ranges;
     const int32_t *nbrs;  // unweighted layout
     const int2 *adj;      // weighted layout: {neighbour, weight bits}
@@ -62,13 +64,16 @@ struct Dev {
     int32_t *req;
     uint8_t *st;
     unsigned long long *resp;  // [2][n], PUSH only
-    int32_t *lists;            // [2][NLISTS][cap]
-    int32_t *ctr;              // [ROUND_SLOTS][CTR_STRIDE]
-    unsigned long long *ctr64; // [ROUND_SLOTS][CTR64_STRIDE]
+    int32_t *lists;            // [2][NLISTS][cap]; classes 0 and 2 split into nb buckets of bcap
+    int32_t *ctr;              // [ROUND_SLOTS][CTR_STRIDE] totals; [1],[3] = hub list sizes (atomics)
+    int32_t *ctrb;             // [ROUND_SLOTS][4][nb] per-bucket: blue-lo, red-lo, matched, dead
+    unsigned long long *ctr64; // [ROUND_SLOTS][CTR64_STRIDE] degree sums (profile)
+    unsigned long long *ctrb64;// [ROUND_SLOTS][2][nb] per-bucket degree sums (profile)
+    unsigned long long *hub64; // [ROUND_SLOTS][2] degree sums of hub rows (profile)
     const uint32_t *seeds;     // seeds[r - 1] drives the select of round r
-    const int32_t *vbase;      // batched: first id of the vertex's graph (else null)
-    const int32_t *vgid;       // batched: graph index of the vertex
-    int32_t *glast;            // batched: last round in which the graph had an active vertex
+    const int32_t *loc;        // graph-local ORIGINAL id of each (internally renumbered) vertex:
+                               // the hash input, the proposal tie-break id and the match value base
+    uint8_t *vround;           // batched: round in which the vertex stopped being active (255: never)
 };
 
 __device__ __forceinline__ int32_t *list_ptr(const Dev &d, int parity, int k)
@@ -121,19 +126,6 @@ __device__ __forceinline__ void wave_append(bool pred, int32_t value, int32_t *l
     if (lane == leader) base = atomicAdd(counter, (int)__popcll(mask));
     base = __shfl(base, leader);
     if (pred) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = value;
-}
-
-__device__ __forceinline__ void wave_count(bool pred, int32_t *counter)
-{
-    const unsigned long long mask = __ballot(pred);
-    if (mask != 0ull && __lane_id() == __ffsll((long long)mask) - 1) atomicAdd(counter, (int)__popcll(mask));
-}
-
-__device__ __forceinline__ void wave_sum64(unsigned long long x, unsigned long long *counter)
-{
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
-    if (__lane_id() == 0 && x != 0ull) atomicAdd(counter, x);
 }
 
 // Candidate of a row scan: best weight, its CSR position and neighbour, and
@@ -199,11 +191,14 @@ template <bool A8>
 __device__ __forceinline__ void load_entry(const Dev &d, int32_t j, int32_t &u, float &w)
 {
     if (A8) {
-        const int2 e = d.adj[j];
+        // Rows are streamed once per round: non-temporal, so they do not evict
+        // the gathered state arrays from L2.
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        const v2i e = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(d.adj + j));
         u = e.x;
         w = __int_as_float(e.y);
     } else {
-        u = d.nbrs[j];
+        u = __builtin_nontemporal_load(d.nbrs + j);
         w = 1.0f;
     }
 }
@@ -272,101 +267,354 @@ __device__ __forceinline__ void write_request(const Dev &d, int32_t v, const Can
 {
     if (c.j != INT_MAX) {
         d.req[v] = c.u;
-        if (PUSH) atomicMax(resp + c.u, proposal_key<W>(c.w, v));
+        if (PUSH) atomicMax(resp + c.u, proposal_key<W>(c.w, d.loc[v]));
     } else {
         d.req[v] = c.alive ? REQ_NONE : REQ_DEAD;
     }
 }
 
-__global__ void k_clear(int32_t *ctr, unsigned long long *ctr64, int n32, int n64)
+__global__ void k_clear(int32_t *ctr, unsigned long long *hub64, int n32, int n64)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n32; i += gridDim.x * blockDim.x) ctr[i] = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n64; i += gridDim.x * blockDim.x) ctr64[i] = 0ull;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n64; i += gridDim.x * blockDim.x) hub64[i] = 0ull;
 }
 
-// Select of round 1 over all vertices (match starts at 0 everywhere,
-// src/matchgpu.cu:486), building round 1's work lists.
+__device__ __forceinline__ int32_t *ctrb_ptr(const Dev &d, int r, int j) { return d.ctrb + ((size_t)r * 4 + j) * d.nb; }
+__device__ __forceinline__ unsigned long long *ctrb64_ptr(const Dev &d, int r, int j)
+{
+    return d.ctrb64 + ((size_t)r * 2 + j) * d.nb;
+}
+
+// Appends value to this block's bucket for every lane with pred set; the
+// bucket counter lives in LDS, so no global atomic is involved.  Must be
+// reached by all lanes of the wavefront.
+__device__ __forceinline__ void bucket_append(bool pred, int32_t value, int32_t *bucket, int32_t *s_count)
+{
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(s_count, (int)__popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) bucket[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Block sum of per-thread values; the result is valid in thread 0.
+template <typename T>
+__device__ T block_sum(T x)
+{
+    __shared__ T s_part[16];
+    x = wave_sum(x);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    T t = 0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)((blockDim.x + 63) >> 6); ++w) t += s_part[w];
+    __syncthreads();
+    return t;
+}
+
+// Select of round 1 (match starts at 0 everywhere, src/matchgpu.cu:486).
+// Block b colours the ids of bucket b and files them into round 1's lists.
 template <bool PUSH>
 __global__ void k_init_select(Dev d)
 {
-    const int parity = 1;
-    int32_t *cnt = d.ctr + 1 * CTR_STRIDE;
+    __shared__ int32_t s_cnt[2];
+    const int parity = 1, b = blockIdx.x;
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t seed = d.seeds[0];
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < d.n;
-         base += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t lo = (int64_t)b * d.bcap, hi = min((int64_t)d.n, lo + d.bcap);
+    int32_t *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
+    int32_t *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
+    unsigned long long deg_sum = 0, hub_deg_sum = 0;
+    for (int64_t base = lo + (threadIdx.x & ~63); base < hi; base += blockDim.x) {
         const int32_t v = (int32_t)(base + (threadIdx.x & 63));
-        const bool valid = v < d.n;
+        const bool valid = v < hi;
         int cls = -1;
-        unsigned long long deg = 0;
         if (valid) {
-            const int32_t vb = d.vbase ? d.vbase[v] : 0;
-            const int c = colour_of((uint32_t)(v - vb), seed, d.barrier);
+            const int c = colour_of((uint32_t)d.loc[v], seed, d.barrier);
             d.match[v] = c;
             d.st[v] = (uint8_t)c;
+            if (d.vround) d.vround[v] = 255;
             const int2 rg = d.ranges[v];
-            deg = (unsigned long long)(rg.y - rg.x);
-            cls = 2 * c + ((rg.y - rg.x) > d.hub_deg ? 1 : 0);
+            const int hub = (rg.y - rg.x) > d.hub_deg ? 1 : 0;
+            cls = 2 * c + hub;
+            if (hub) hub_deg_sum += (unsigned long long)(rg.y - rg.x);
+            else deg_sum += (unsigned long long)(rg.y - rg.x);
             if (PUSH && c == ST_RED) d.resp[(size_t)parity * d.n + v] = 0ull;
         }
-#pragma unroll
-        for (int k = 0; k < NLISTS; ++k) wave_append(cls == k, v, list_ptr(d, parity, k), cnt + k);
-        if (d.count_nnz) wave_sum64(deg, d.ctr64 + 1 * CTR64_STRIDE + 0);
+        bucket_append(cls == L_BLUE_LO, v, blue, s_cnt + 0);
+        bucket_append(cls == L_RED_LO, v, red, s_cnt + 1);
+        wave_append(cls == L_BLUE_HI, v, list_ptr(d, parity, L_BLUE_HI), d.ctr + 1 * CTR_STRIDE + L_BLUE_HI);
+        wave_append(cls == L_RED_HI, v, list_ptr(d, parity, L_RED_HI), d.ctr + 1 * CTR_STRIDE + L_RED_HI);
+    }
+    if (d.count_nnz) {
+        deg_sum = block_sum(deg_sum);
+        hub_deg_sum = block_sum(hub_deg_sum);
+        if (threadIdx.x == 0) {
+            ctrb64_ptr(d, 1, 0)[b] = deg_sum;
+            if (hub_deg_sum) atomicAdd(d.hub64 + 1 * 2 + 0, hub_deg_sum);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctrb_ptr(d, 1, 0)[b] = s_cnt[0];
+        ctrb_ptr(d, 1, 1)[b] = s_cnt[1];
     }
 }
 
-// Request phase of round r.  Blocks [0, hub_blocks) take the hub rows one
-// block per row; the rest take the other blue rows, G lanes per row.
+// Request phase of round r.  Blocks [0, nb) take bucket b of the blue rows,
+// G lanes per row; blocks [nb, nb + hub_blocks) take the hub rows, one block
+// per row.
 template <bool W, bool A8, bool PUSH, int G>
 __global__ void k_request(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
-    const int32_t *cnt = d.ctr + r * CTR_STRIDE;
     unsigned long long *resp = PUSH ? d.resp + (size_t)parity * d.n : nullptr;
-    unsigned long long *blue_nnz = d.ctr64 + r * CTR64_STRIDE + 1;
-    if ((int)blockIdx.x < hub_blocks) {
-        const int32_t count = cnt[L_BLUE_HI];
+    if ((int)blockIdx.x >= d.nb) {
+        const int hb = blockIdx.x - d.nb;
+        const int32_t count = d.ctr[r * CTR_STRIDE + L_BLUE_HI];
         const int32_t *list = list_ptr(d, parity, L_BLUE_HI);
-        for (int32_t idx = blockIdx.x; idx < count; idx += hub_blocks) {
+        unsigned long long nnz = 0;
+        for (int32_t idx = hb; idx < count; idx += hub_blocks) {
             const int32_t v = list[idx];
             const int2 rg = d.ranges[v];
             Cand c = scan_request<W, A8>(d, rg, threadIdx.x, blockDim.x);
             c = cand_block_reduce<W>(c);
             if (threadIdx.x == 0) {
                 write_request<W, PUSH>(d, v, c, resp);
-                if (d.count_nnz) atomicAdd(blue_nnz, (unsigned long long)(rg.y - rg.x));
+                nnz += (unsigned long long)(rg.y - rg.x);
             }
         }
+        if (d.count_nnz && threadIdx.x == 0 && nnz) atomicAdd(d.hub64 + (size_t)r * 2 + 1, nnz);
         return;
     }
-    const int32_t count = cnt[L_BLUE_LO];
-    const int32_t *list = list_ptr(d, parity, L_BLUE_LO);
+    const int b = blockIdx.x;
+    const int32_t count = ctrb_ptr(d, r, 0)[b];
+    const int32_t *list = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
     constexpr int GPW = 64 / G;  // groups per wavefront
     const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
-    const int waves_per_block = blockDim.x >> 6;
-    const int64_t wave0 = (int64_t)(blockIdx.x - hub_blocks) * waves_per_block + (threadIdx.x >> 6);
-    const int64_t stride = (int64_t)(gridDim.x - hub_blocks) * waves_per_block * GPW;
-    for (int64_t first = wave0 * GPW; first < count; first += stride) {
-        const int64_t idx = first + gw;
+    const int stride = (blockDim.x >> 6) * GPW;
+    unsigned long long nnz = 0;
+    for (int first = (threadIdx.x >> 6) * GPW; first < count; first += stride) {
+        const int idx = first + gw;
         const bool valid = idx < count;
         const int32_t v = valid ? list[idx] : 0;
         const int2 rg = valid ? d.ranges[v] : make_int2(0, 0);
         Cand c = scan_request<W, A8>(d, rg, gl, G);
         if (G > 1) c = cand_shfl_reduce<W>(c, G);
-        if (valid && gl == 0) write_request<W, PUSH>(d, v, c, resp);
-        if (d.count_nnz) wave_sum64(valid && gl == 0 ? (unsigned long long)(rg.y - rg.x) : 0ull, blue_nnz);
+        if (valid && gl == 0) {
+            write_request<W, PUSH>(d, v, c, resp);
+            nnz += (unsigned long long)(rg.y - rg.x);
+        }
+    }
+    if (d.count_nnz) {
+        nnz = block_sum(nnz);
+        if (threadIdx.x == 0) ctrb64_ptr(d, r, 1)[b] = nnz;
     }
 }
 
-// Respond phase of round r (PULL): red vertices pick their proposer.
-template <bool W, bool A8, int G>
-__global__ void k_respond(Dev d, int r, int hub_blocks)
+// ---------------------------------------------------------------------------
+// Entry-parallel tiles.  A block takes up to blockDim vertices of its bucket,
+// stages their ids and row starts in LDS with an exclusive scan of the row
+// lengths, and then hands out ROW ENTRIES, not rows, to its threads: each
+// thread finds the row of entry e by binary search over the LDS prefix and
+// issues U independent row loads before using any.  The per-row winner is an
+// LDS 64-bit atomic max on (order-preserving weight, ~position in row), i.e.
+// largest weight, first in CSR order on ties -- the reference's strict '>'
+// scan from -1.0.  Only two dependent memory steps remain per entry (row
+// entry, neighbour state), whatever the degree distribution.
+// ---------------------------------------------------------------------------
+constexpr int TILE_U = 4;  // row entries each thread has in flight
+
+// Exclusive block scan of x; total in *s_total.  Reached by all threads.
+__device__ int32_t block_exclusive_scan(int32_t x, int32_t *s_wave, int32_t *s_total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    int32_t incl = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int32_t y = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += y;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t acc = 0;
+        for (int w = 0; w < nw; ++w) {
+            const int32_t t = s_wave[w];
+            s_wave[w] = acc;
+            acc += t;
+        }
+        *s_total = acc;
+    }
+    __syncthreads();
+    return s_wave[wave] + incl - x;
+}
+
+// Largest k in [0, tn) with off[k] <= e (off[0] == 0 <= e).
+__device__ __forceinline__ int tile_row_of(const int32_t *off, int tn, int32_t e)
+{
+    int lo = 0, hi = tn - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t order_bits(float w)
+{
+    const uint32_t bits = __float_as_uint(w == 0.0f ? 0.0f : w);
+    return (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u);
+}
+
+// RESPOND == false: request scan of blue rows; true: respond scan of red rows.
+template <bool W, bool A8, bool PUSH, bool RESPOND>
+__device__ void tile_scan_bucket(const Dev &d, const int32_t *list, int32_t count, unsigned long long *resp)
+{
+    extern __shared__ unsigned long long s_dyn[];
+    const int T = blockDim.x;
+    unsigned long long *s_key = s_dyn;                          // [T]
+    int32_t *s_v = reinterpret_cast<int32_t *>(s_key + T);      // [T]
+    int32_t *s_beg = s_v + T;                                   // [T]
+    int32_t *s_off = s_beg + T;                                 // [T]
+    int32_t *s_alive = s_off + T;                               // [T]
+    int32_t *s_wave = s_alive + T;                              // [16]
+    int32_t *s_total = s_wave + 16;                             // [1]
+    const int tid = threadIdx.x;
+    for (int t0 = 0; t0 < count; t0 += T) {
+        const int tn = min(T, count - t0);
+        int32_t deg = 0;
+        if (tid < tn) {
+            const int32_t v = __builtin_nontemporal_load(list + t0 + tid);
+            const int2 rg = d.ranges[v];
+            s_v[tid] = v;
+            s_beg[tid] = rg.x;
+            deg = rg.y - rg.x;
+            s_key[tid] = 0ull;
+            s_alive[tid] = 0;
+        }
+        s_off[tid] = block_exclusive_scan(deg, s_wave, s_total);
+        __syncthreads();
+        const int32_t E = *s_total;
+        for (int32_t e0 = 0; e0 < E; e0 += T * TILE_U) {
+            int32_t u[TILE_U], k[TILE_U], pos[TILE_U];
+            float w[TILE_U];
+#pragma unroll
+            for (int q = 0; q < TILE_U; ++q) {
+                const int32_t e = e0 + q * T + tid;
+                k[q] = -1;
+                if (e < E) {
+                    k[q] = tile_row_of(s_off, tn, e);
+                    pos[q] = e - s_off[k[q]];
+                    load_entry<A8>(d, s_beg[k[q]] + pos[q], u[q], w[q]);
+                }
+            }
+            uint8_t s[TILE_U];
+            int32_t rq[TILE_U];
+#pragma unroll
+            for (int q = 0; q < TILE_U; ++q) {
+                s[q] = k[q] >= 0 ? d.st[u[q]] : ST_MATCHED;
+                if (RESPOND) rq[q] = (k[q] >= 0 && s[q] == ST_BLUE) ? d.req[u[q]] : -1;
+            }
+#pragma unroll
+            for (int q = 0; q < TILE_U; ++q) {
+                if (k[q] < 0) continue;
+                bool cand;
+                if (RESPOND) {
+                    cand = s[q] == ST_BLUE && rq[q] == s_v[k[q]];
+                } else {
+                    if (s[q] != ST_MATCHED) s_alive[k[q]] = 1;
+                    cand = s[q] == ST_RED;
+                }
+                if (W) cand = cand && w[q] > -1.0f;
+                if (cand) {
+                    const unsigned long long key = ((unsigned long long)(W ? order_bits(w[q]) : 1u) << 32) |
+                                                   (unsigned long long)(0xFFFFFFFFu - (uint32_t)pos[q]);
+                    atomicMax(s_key + k[q], key);
+                }
+            }
+        }
+        __syncthreads();
+        if (tid < tn) {
+            const int32_t v = s_v[tid];
+            const unsigned long long key = s_key[tid];
+            if (key != 0ull) {
+                const int32_t j = s_beg[tid] + (int32_t)(0xFFFFFFFFu - (uint32_t)key);
+                int32_t un;
+                float wn;
+                load_entry<A8>(d, j, un, wn);
+                d.req[v] = un;
+                if (!RESPOND && PUSH)
+                    atomicMax(resp + un,
+                              (key & 0xFFFFFFFF00000000ull) | (unsigned long long)(0xFFFFFFFFu - (uint32_t)d.loc[v]));
+            } else {
+                d.req[v] = RESPOND ? REQ_NONE : (s_alive[tid] ? REQ_NONE : REQ_DEAD);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <bool W, bool A8, bool PUSH>
+__global__ void k_request_tile(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
-    const int32_t *cnt = d.ctr + r * CTR_STRIDE;
-    if ((int)blockIdx.x < hub_blocks) {
-        const int32_t count = cnt[L_RED_HI];
+    unsigned long long *resp = PUSH ? d.resp + (size_t)parity * d.n : nullptr;
+    if ((int)blockIdx.x >= d.nb) {
+        const int hb = blockIdx.x - d.nb;
+        const int32_t count = d.ctr[r * CTR_STRIDE + L_BLUE_HI];
+        const int32_t *list = list_ptr(d, parity, L_BLUE_HI);
+        unsigned long long nnz = 0;
+        for (int32_t idx = hb; idx < count; idx += hub_blocks) {
+            const int32_t v = list[idx];
+            const int2 rg = d.ranges[v];
+            Cand c = scan_request<W, A8>(d, rg, threadIdx.x, blockDim.x);
+            c = cand_block_reduce<W>(c);
+            if (threadIdx.x == 0) {
+                write_request<W, PUSH>(d, v, c, resp);
+                nnz += (unsigned long long)(rg.y - rg.x);
+            }
+        }
+        if (d.count_nnz && threadIdx.x == 0 && nnz) atomicAdd(d.hub64 + (size_t)r * 2 + 1, nnz);
+        return;
+    }
+    const int b = blockIdx.x;
+    const int32_t count = ctrb_ptr(d, r, 0)[b];
+    const int32_t *list = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
+    tile_scan_bucket<W, A8, PUSH, false>(d, list, count, resp);
+    if (d.count_nnz) {
+        unsigned long long nnz = 0;
+        for (int i = threadIdx.x; i < count; i += blockDim.x) {
+            const int2 rg = d.ranges[list[i]];
+            nnz += (unsigned long long)(rg.y - rg.x);
+        }
+        nnz = block_sum(nnz);
+        if (threadIdx.x == 0) ctrb64_ptr(d, r, 1)[b] = nnz;
+    }
+}
+
+template <bool W, bool A8>
+__global__ void k_respond_tile(Dev d, int r, int hub_blocks)
+{
+    const int parity = r & 1;
+    if ((int)blockIdx.x >= d.nb) {
+        const int hb = blockIdx.x - d.nb;
+        const int32_t count = d.ctr[r * CTR_STRIDE + L_RED_HI];
         const int32_t *list = list_ptr(d, parity, L_RED_HI);
-        for (int32_t idx = blockIdx.x; idx < count; idx += hub_blocks) {
+        for (int32_t idx = hb; idx < count; idx += hub_blocks) {
             const int32_t v = list[idx];
             const int2 rg = d.ranges[v];
             Cand c = scan_respond<W, A8>(d, v, rg, threadIdx.x, blockDim.x);
@@ -375,15 +623,39 @@ __global__ void k_respond(Dev d, int r, int hub_blocks)
         }
         return;
     }
-    const int32_t count = cnt[L_RED_LO];
-    const int32_t *list = list_ptr(d, parity, L_RED_LO);
+    const int b = blockIdx.x;
+    tile_scan_bucket<W, A8, false, true>(d, list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap,
+                                         ctrb_ptr(d, r, 1)[b], nullptr);
+}
+
+__host__ __device__ constexpr size_t tile_smem_bytes(int T) { return (size_t)T * (8 + 4 * 4) + 17 * 4; }
+
+// Respond phase of round r (PULL): red vertices pick their proposer.
+template <bool W, bool A8, int G>
+__global__ void k_respond(Dev d, int r, int hub_blocks)
+{
+    const int parity = r & 1;
+    if ((int)blockIdx.x >= d.nb) {
+        const int hb = blockIdx.x - d.nb;
+        const int32_t count = d.ctr[r * CTR_STRIDE + L_RED_HI];
+        const int32_t *list = list_ptr(d, parity, L_RED_HI);
+        for (int32_t idx = hb; idx < count; idx += hub_blocks) {
+            const int32_t v = list[idx];
+            const int2 rg = d.ranges[v];
+            Cand c = scan_respond<W, A8>(d, v, rg, threadIdx.x, blockDim.x);
+            c = cand_block_reduce<W>(c);
+            if (threadIdx.x == 0) d.req[v] = c.j != INT_MAX ? c.u : REQ_NONE;
+        }
+        return;
+    }
+    const int b = blockIdx.x;
+    const int32_t count = ctrb_ptr(d, r, 1)[b];
+    const int32_t *list = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
     constexpr int GPW = 64 / G;
     const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
-    const int waves_per_block = blockDim.x >> 6;
-    const int64_t wave0 = (int64_t)(blockIdx.x - hub_blocks) * waves_per_block + (threadIdx.x >> 6);
-    const int64_t stride = (int64_t)(gridDim.x - hub_blocks) * waves_per_block * GPW;
-    for (int64_t first = wave0 * GPW; first < count; first += stride) {
-        const int64_t idx = first + gw;
+    const int stride = (blockDim.x >> 6) * GPW;
+    for (int first = (threadIdx.x >> 6) * GPW; first < count; first += stride) {
+        const int idx = first + gw;
         const bool valid = idx < count;
         const int32_t v = valid ? list[idx] : 0;
         const int2 rg = valid ? d.ranges[v] : make_int2(0, 0);
@@ -393,100 +665,283 @@ __global__ void k_respond(Dev d, int r, int hub_blocks)
     }
 }
 
-// Match write-back of round r fused with the select of round r+1 (when
-// do_select): resolves every vertex of round r's lists, recolours the ones
-// still active and appends them to round r+1's lists.
+// Writes the outcome of round r for vertex v (gMatch, src/matchgpu.cu:260-283)
+// and, when do_select, its colour for round r+1 (gSelect).  Returns the class
+// of v in round r+1, or -1 when it left the active set.
+// partner is the partner's graph-local original id (-1: none); vloc is v's.
+template <bool PUSH>
+__device__ __forceinline__ int settle_vertex(const Dev &d, int32_t v, int32_t vloc, int colour, int hub,
+                                             int32_t partner, bool dead, int r, int do_select, uint32_t seed,
+                                             unsigned long long *nresp)
+{
+    if (partner >= 0) {
+        d.match[v] = 4 + min(vloc, partner);
+        d.st[v] = ST_MATCHED;
+        if (d.vround) d.vround[v] = (uint8_t)r;
+        return -1;
+    }
+    if (dead) {
+        d.match[v] = 2;
+        d.st[v] = ST_DEAD;
+        if (d.vround) d.vround[v] = (uint8_t)r;
+        return -1;
+    }
+    if (!do_select) return -1;
+    const int c = colour_of((uint32_t)vloc, seed, d.barrier);
+    if (c != colour) {
+        d.match[v] = c;
+        d.st[v] = (uint8_t)c;
+    }
+    if (PUSH && c == ST_RED) nresp[v] = 0ull;
+    return 2 * c + hub;
+}
+
+// Partner of v after round r's responses, as a graph-local original id, from
+// its request a = req[v], the word k read for it (PUSH: the proposal key at the
+// red end, whose low half is the winner's local id; PULL: req[] of the other
+// end) and pl = loc[a].  -1 when unmatched; dead flags a blue vertex without
+// unmatched neighbours.
+template <bool PUSH>
+__device__ __forceinline__ int32_t partner_of(int32_t v, int32_t vloc, int colour, int32_t a, unsigned long long k,
+                                              int32_t pl, bool &dead)
+{
+    if (colour == ST_BLUE) {
+        if (a == REQ_DEAD) {
+            dead = true;
+            return -1;
+        }
+        if (a < 0) return -1;
+        if (PUSH) return (uint32_t)k == 0xFFFFFFFFu - (uint32_t)vloc ? pl : -1;
+        return (int32_t)k == v ? pl : -1;
+    }
+    if (PUSH) return k != 0ull ? (int32_t)(0xFFFFFFFFu - (uint32_t)k) : -1;
+    return (a >= 0 && (int32_t)k == v) ? pl : -1;
+}
+
+// Loads of partner_of's inputs for vertex v.
+template <bool PUSH>
+__device__ __forceinline__ void partner_words(const Dev &d, int32_t v, int colour, const unsigned long long *resp,
+                                              int32_t &a, unsigned long long &k)
+{
+    a = -1;
+    k = 0ull;
+    if (PUSH && colour == ST_RED) k = resp[v];
+    else a = d.req[v];
+}
+template <bool PUSH>
+__device__ __forceinline__ void partner_words2(const Dev &d, int colour, const unsigned long long *resp, int32_t a,
+                                               unsigned long long &k, int32_t &pl)
+{
+    pl = -1;
+    if (a < 0) return;
+    if (PUSH) {
+        if (colour == ST_BLUE) k = resp[a];
+    } else {
+        k = (unsigned long long)(uint32_t)d.req[a];
+    }
+    pl = d.loc[a];
+}
+
+// Match write-back of round r fused with the select of round r+1: block b
+// resolves bucket b of round r's lists (and a share of the hub lists),
+// recolours the vertices still active and files them into bucket b of round
+// r+1's lists.
 template <bool PUSH>
 __global__ void k_match_select(Dev d, int r, int do_select)
 {
-    const int parity = r & 1, nparity = parity ^ 1;
-    int32_t *cnt = d.ctr + r * CTR_STRIDE;
-    int32_t *ncnt = d.ctr + (r + 1) * CTR_STRIDE;
-    const int32_t c0 = cnt[0], c1 = cnt[1], c2 = cnt[2], c3 = cnt[3];
-    const int64_t total = (int64_t)c0 + c1 + c2 + c3;
+    __shared__ int32_t s_cnt[2];
+    const int parity = r & 1, nparity = parity ^ 1, b = blockIdx.x;
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int32_t c0 = ctrb_ptr(d, r, 0)[b], c1 = ctrb_ptr(d, r, 1)[b];
+    const int32_t total = c0 + c1;
+    const int32_t *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
+    const int32_t *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
+    int32_t *nblue = list_ptr(d, nparity, L_BLUE_LO) + (size_t)b * d.bcap;
+    int32_t *nred = list_ptr(d, nparity, L_RED_LO) + (size_t)b * d.bcap;
     const unsigned long long *resp = PUSH ? d.resp + (size_t)parity * d.n : nullptr;
     unsigned long long *nresp = PUSH ? d.resp + (size_t)nparity * d.n : nullptr;
     const uint32_t seed = do_select ? d.seeds[r] : 0u;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t t = base + (threadIdx.x & 63);
-        const bool valid = t < total;
-        int cls = -1;
-        bool now_matched = false, now_dead = false;
-        int32_t v = 0;
-        unsigned long long deg = 0;
-        if (valid) {
-            int k;
-            int64_t idx = t;
-            if (idx < c0) k = 0;
-            else if ((idx -= c0) < c1) k = 1;
-            else if ((idx -= c1) < c2) k = 2;
-            else { idx -= c2; k = 3; }
-            v = list_ptr(d, parity, k)[idx];
-            const int colour = k >> 1, hub = k & 1;
-            int32_t partner = -1;
-            if (colour == ST_BLUE) {
-                const int32_t rq = d.req[v];
-                if (rq == REQ_DEAD) now_dead = true;
-                else if (rq >= 0) {
-                    if (PUSH) {
-                        if ((uint32_t)resp[rq] == 0xFFFFFFFFu - (uint32_t)v) partner = rq;
-                    } else if (d.req[rq] == v) partner = rq;
-                }
-            } else {
-                if (PUSH) {
-                    const unsigned long long key = resp[v];
-                    if (key != 0ull) partner = (int32_t)(0xFFFFFFFFu - (uint32_t)key);
-                } else {
-                    const int32_t rq = d.req[v];
-                    if (rq >= 0 && d.req[rq] == v) partner = rq;
-                }
-            }
-            const int32_t vb = d.vbase ? d.vbase[v] : 0;
-            if (partner >= 0) {
-                now_matched = true;
-                d.match[v] = 4 + min(v, partner) - vb;
-                d.st[v] = ST_MATCHED;
-            } else if (now_dead) {
-                d.match[v] = 2;
-                d.st[v] = ST_DEAD;
-            } else if (do_select) {
-                const int c = colour_of((uint32_t)(v - vb), seed, d.barrier);
-                if (c != colour) {
-                    d.match[v] = c;
-                    d.st[v] = (uint8_t)c;
-                }
-                if (PUSH && c == ST_RED) nresp[v] = 0ull;
-                cls = 2 * c + hub;
-                if (d.count_nnz) {
-                    const int2 rg = d.ranges[v];
-                    deg = (unsigned long long)(rg.y - rg.x);
-                }
-                if (d.glast) atomicMax(d.glast + d.vgid[v], r + 1);
+    int32_t n_matched = 0, n_dead = 0;
+    unsigned long long deg_sum = 0, hub_deg_sum = 0;
+    // MU vertices per thread: all loads of a step are issued before any store.
+    constexpr int MU = 4;
+    for (int base = threadIdx.x & ~63; base < total; base += blockDim.x * MU) {
+        int32_t v[MU], a[MU], vl[MU], pl[MU];
+        unsigned long long k[MU];
+        int col[MU];
+#pragma unroll
+        for (int q = 0; q < MU; ++q) {
+            const int t = base + q * blockDim.x + (threadIdx.x & 63);
+            col[q] = -1;
+            v[q] = 0;
+            if (t < total) {
+                col[q] = t < c0 ? ST_BLUE : ST_RED;
+                v[q] = __builtin_nontemporal_load(t < c0 ? blue + t : red + (t - c0));
             }
         }
-        wave_count(now_matched, cnt + 4);
-        wave_count(now_dead, cnt + 5);
-        if (do_select) {
 #pragma unroll
-            for (int k = 0; k < NLISTS; ++k) wave_append(cls == k, v, list_ptr(d, nparity, k), ncnt + k);
-            if (d.count_nnz) wave_sum64(deg, d.ctr64 + (r + 1) * CTR64_STRIDE + 0);
+        for (int q = 0; q < MU; ++q) {
+            a[q] = -1;
+            k[q] = 0ull;
+            vl[q] = 0;
+            if (col[q] >= 0) {
+                partner_words<PUSH>(d, v[q], col[q], resp, a[q], k[q]);
+                vl[q] = d.loc[v[q]];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < MU; ++q) {
+            pl[q] = -1;
+            if (col[q] >= 0) partner_words2<PUSH>(d, col[q], resp, a[q], k[q], pl[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < MU; ++q) {
+            int cls = -1;
+            if (col[q] >= 0) {
+                bool dd = false;
+                const int32_t partner = partner_of<PUSH>(v[q], vl[q], col[q], a[q], k[q], pl[q], dd);
+                n_matched += partner >= 0;
+                n_dead += dd;
+                cls = settle_vertex<PUSH>(d, v[q], vl[q], col[q], 0, partner, dd, r, do_select, seed, nresp);
+                if (cls >= 0 && d.count_nnz) {
+                    const int2 rg = d.ranges[v[q]];
+                    deg_sum += (unsigned long long)(rg.y - rg.x);
+                }
+            }
+            bucket_append(cls == L_BLUE_LO, v[q], nblue, s_cnt + 0);
+            bucket_append(cls == L_RED_LO, v[q], nred, s_cnt + 1);
+        }
+    }
+    // hub rows: a share of the global hub lists
+    const int32_t h0 = d.ctr[r * CTR_STRIDE + L_BLUE_HI], h1 = d.ctr[r * CTR_STRIDE + L_RED_HI];
+    const int32_t *hblue = list_ptr(d, parity, L_BLUE_HI), *hred = list_ptr(d, parity, L_RED_HI);
+    for (int64_t base = (int64_t)b * blockDim.x + (threadIdx.x & ~63); base < (int64_t)h0 + h1;
+         base += (int64_t)d.nb * blockDim.x) {
+        const int64_t t = base + (threadIdx.x & 63);
+        int cls = -1;
+        int32_t v = 0;
+        if (t < (int64_t)h0 + h1) {
+            const int colour = t < h0 ? ST_BLUE : ST_RED;
+            v = t < h0 ? hblue[t] : hred[t - h0];
+            int32_t a, pl;
+            unsigned long long k;
+            partner_words<PUSH>(d, v, colour, resp, a, k);
+            partner_words2<PUSH>(d, colour, resp, a, k, pl);
+            const int32_t vloc = d.loc[v];
+            bool dd = false;
+            const int32_t partner = partner_of<PUSH>(v, vloc, colour, a, k, pl, dd);
+            n_matched += partner >= 0;
+            n_dead += dd;
+            cls = settle_vertex<PUSH>(d, v, vloc, colour, 1, partner, dd, r, do_select, seed, nresp);
+            if (cls >= 0 && d.count_nnz) {
+                const int2 rg = d.ranges[v];
+                hub_deg_sum += (unsigned long long)(rg.y - rg.x);
+            }
+        }
+        wave_append(cls == L_BLUE_HI, v, list_ptr(d, nparity, L_BLUE_HI), d.ctr + (r + 1) * CTR_STRIDE + L_BLUE_HI);
+        wave_append(cls == L_RED_HI, v, list_ptr(d, nparity, L_RED_HI), d.ctr + (r + 1) * CTR_STRIDE + L_RED_HI);
+    }
+    n_matched = block_sum(n_matched);
+    n_dead = block_sum(n_dead);
+    if (d.count_nnz) {
+        deg_sum = block_sum(deg_sum);
+        hub_deg_sum = block_sum(hub_deg_sum);
+        if (threadIdx.x == 0 && do_select) {
+            ctrb64_ptr(d, r + 1, 0)[b] = deg_sum;
+            if (hub_deg_sum) atomicAdd(d.hub64 + (size_t)(r + 1) * 2 + 0, hub_deg_sum);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ctrb_ptr(d, r, 2)[b] = n_matched;
+        ctrb_ptr(d, r, 3)[b] = n_dead;
+        if (do_select) {
+            ctrb_ptr(d, r + 1, 0)[b] = s_cnt[0];
+            ctrb_ptr(d, r + 1, 1)[b] = s_cnt[1];
         }
     }
 }
 
-// Builds the interleaved {neighbour, weight} rows of the weighted layout.
-__global__ void k_interleave(int2 *adj, const int32_t *nbrs, const float *w, int64_t nnz)
+// Sums the per-bucket counters of rounds [r0, r0 + gridDim.x) into the
+// per-round totals (one block per round).  Assigns, so it may be re-run.
+__global__ void k_reduce_counts(Dev d, int r0, int with_nnz)
 {
-    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nnz; j += (int64_t)gridDim.x * blockDim.x)
-        adj[j] = make_int2(nbrs[j], __float_as_int(w[j]));
+    const int r = r0 + blockIdx.x;
+    int32_t a[4] = {0, 0, 0, 0};
+    unsigned long long z[2] = {0ull, 0ull};
+    for (int b = threadIdx.x; b < d.nb; b += blockDim.x) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] += ctrb_ptr(d, r, j)[b];
+        if (with_nnz) {
+            z[0] += ctrb64_ptr(d, r, 0)[b];
+            z[1] += ctrb64_ptr(d, r, 1)[b];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a[j] = block_sum(a[j]);
+    if (with_nnz) {
+        z[0] = block_sum(z[0]);
+        z[1] = block_sum(z[1]);
+    }
+    if (threadIdx.x == 0) {
+        int32_t *c = d.ctr + (size_t)r * CTR_STRIDE;
+        c[L_BLUE_LO] = a[0];
+        c[L_RED_LO] = a[1];
+        c[4] = a[2];
+        c[5] = a[3];
+        if (with_nnz) {
+            d.ctr64[(size_t)r * CTR64_STRIDE + 0] = z[0] + d.hub64[(size_t)r * 2 + 0];
+            d.ctr64[(size_t)r * CTR64_STRIDE + 1] = z[1] + d.hub64[(size_t)r * 2 + 1];
+        }
+    }
 }
+
+// Writes the state array back in the caller's vertex numbering.
+__global__ void k_unpermute(const int32_t *match, const int32_t *old_of, int32_t *out, int32_t n)
+{
+    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
+        out[old_of[v]] = match[v];
+}
+
+// Rounds each batched graph would execute alone: one plus the last round in
+// which one of its vertices was still active (src/matchgpu.cu:604 loop).
+__global__ void k_graph_rounds(const uint8_t *vround, const int32_t *voff, int32_t n_graphs, int32_t R_run,
+                               int32_t R_max, int32_t maximal, int32_t *out)
+{
+    const int gidx = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (gidx >= n_graphs) return;
+    const int32_t v0 = voff[gidx], v1 = voff[gidx + 1];
+    int32_t x = 0;
+    for (int32_t v = v0 + (threadIdx.x & 63); v < v1; v += 64) x = max(x, (int32_t)vround[v]);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x = max(x, __shfl_xor(x, off, 64));
+    if ((threadIdx.x & 63) == 0) {
+        int32_t rounds;
+        if (!maximal) rounds = R_max;
+        else if (v1 == v0) rounds = 1;
+        else if (x == 255) rounds = R_run;
+        else rounds = min(R_max, x + 1);
+        out[gidx] = rounds;
+    }
+}
+
 
 }  // namespace
 
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
+
+// Key of a captured launch sequence: every value baked into its kernel arguments.
+struct ExecKey {
+    int32_t R_run, W, A8, PUSH, G, block, hub_deg, nb, bcap, maximal, hub_blocks, count_nnz, r0, r1;
+    uint32_t barrier;
+};
+struct CachedExec {
+    ExecKey key;
+    hipGraphExec_t exec;
+};
 
 struct mm_graph {
     int device = 0;
@@ -509,17 +964,26 @@ struct mm_graph {
     unsigned long long *d_resp = nullptr;
     int32_t *d_lists = nullptr;
     int32_t *d_ctr = nullptr;
+    int32_t *d_ctrb = nullptr;
     unsigned long long *d_ctr64 = nullptr;
+    unsigned long long *d_ctrb64 = nullptr;
+    unsigned long long *d_hub64 = nullptr;
+    int32_t nb_max = 1;       // bucket slots allocated per counter row
+    int64_t list_cap = 0;     // capacity of one work list
     uint32_t *d_seeds = nullptr;
     int32_t *h_ctr = nullptr;               // pinned
     unsigned long long *h_ctr64 = nullptr;  // pinned
     // batched graphs
     int32_t n_graphs = 0;
-    int32_t *d_vbase = nullptr;
-    int32_t *d_vgid = nullptr;
-    int32_t *d_glast = nullptr;
-    std::vector<int32_t> h_glast_init;
+    int32_t *d_loc = nullptr;        // graph-local original id per internal vertex
+    int32_t *d_old_of = nullptr;     // original (global) id per internal vertex, when relabelled
+    int32_t *d_match_out = nullptr;  // result in the caller's numbering, when relabelled
+    bool relabelled = false;
+    int32_t *d_voff = nullptr;
+    uint8_t *d_vround = nullptr;
+    int32_t *d_grounds = nullptr;
     std::vector<uint32_t> last_seeds;  // seed table currently on the device
+    std::vector<CachedExec> execs;     // captured launch sequences
     hipEvent_t ev_t1 = nullptr, ev_t2 = nullptr;
     std::vector<hipEvent_t> prof_ev;
     int64_t device_bytes = 0;
@@ -559,8 +1023,10 @@ void release(mm_graph *g)
     if (!g) return;
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
-    void *ptrs[] = {g->d_ranges, g->d_nbrs, g->d_adj,  g->d_match, g->d_req,   g->d_st,    g->d_resp,
-                    g->d_lists,  g->d_ctr,  g->d_ctr64, g->d_seeds, g->d_vbase, g->d_vgid, g->d_glast};
+    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_match, g->d_req,   g->d_st,
+                    g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
+                    g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
+                    g->d_old_of, g->d_match_out};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (g->h_ctr) hipHostFree(g->h_ctr);
@@ -568,6 +1034,7 @@ void release(mm_graph *g)
     if (g->ev_t1) hipEventDestroy(g->ev_t1);
     if (g->ev_t2) hipEventDestroy(g->ev_t2);
     for (hipEvent_t e : g->prof_ev) hipEventDestroy(e);
+    for (CachedExec &ce : g->execs) hipGraphExecDestroy(ce.exec);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
 }
@@ -623,8 +1090,37 @@ int32_t analyse_csr(mm_graph *g, const int32_t *ranges, const int32_t *nbrs, con
     return MM_OK;
 }
 
+// Breadth-first order of the vertices, components taken in increasing id
+// order: neighbours get nearby positions, so the per-round gathers of vertex
+// state hit the same cache lines.  order[new] = old.  Components never mix,
+// so every batched graph keeps one contiguous block of new ids, in order.
+std::vector<int32_t> bfs_order(int32_t n, const int32_t *ranges, const int32_t *nbrs)
+{
+    std::vector<int32_t> order((size_t)n);
+    std::vector<uint8_t> seen((size_t)n, 0);
+    size_t head = 0, tail = 0;
+    for (int32_t s0 = 0; s0 < n; ++s0) {
+        if (seen[(size_t)s0]) continue;
+        seen[(size_t)s0] = 1;
+        order[tail++] = s0;
+        while (head < tail) {
+            const int32_t v = order[head++];
+            for (int32_t j = ranges[2 * (size_t)v]; j < ranges[2 * (size_t)v + 1]; ++j) {
+                const int32_t u = nbrs[j];
+                if (!seen[(size_t)u]) {
+                    seen[(size_t)u] = 1;
+                    order[tail++] = u;
+                }
+            }
+        }
+    }
+    return order;
+}
+
+// vbase_old: per ORIGINAL vertex, the first id of its batched graph (null for
+// a single graph).  relabel: renumber the vertices internally (bfs_order).
 int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *nbrs,
-                     const float *w, mm_graph **out)
+                     const float *w, const int32_t *vbase_old, int relabel, mm_graph **out)
 {
     mm::clear_error();
     if (!out) return mm::set_error(MM_ERR_INVALID, "mm_graph_create: out is NULL");
@@ -669,50 +1165,89 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     if ((rc = dev_alloc(g, &g->d_req, nn, "requests")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_st, nn, "state")) != MM_OK) return fail(rc);
     if (g->push_ok_struct && (rc = dev_alloc(g, &g->d_resp, 2 * nn, "proposal keys")) != MM_OK) return fail(rc);
-    if ((rc = dev_alloc(g, &g->d_lists, 2 * NLISTS * nn, "work lists")) != MM_OK) return fail(rc);
+    // Buckets: at most one per block of the smallest block size (64 threads).
+    g->nb_max = std::max(1, g->num_cu * 32);
+    g->list_cap = (int64_t)n + g->nb_max;
+    if ((rc = dev_alloc(g, &g->d_lists, 2 * NLISTS * (size_t)g->list_cap, "work lists")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_ctr, (size_t)ROUND_SLOTS * CTR_STRIDE, "counters")) != MM_OK) return fail(rc);
+    if ((rc = dev_alloc(g, &g->d_ctrb, (size_t)ROUND_SLOTS * 4 * g->nb_max, "bucket counters")) != MM_OK)
+        return fail(rc);
     if ((rc = dev_alloc(g, &g->d_ctr64, (size_t)ROUND_SLOTS * CTR64_STRIDE, "counters")) != MM_OK) return fail(rc);
+    if ((rc = dev_alloc(g, &g->d_ctrb64, (size_t)ROUND_SLOTS * 2 * g->nb_max, "bucket counters")) != MM_OK)
+        return fail(rc);
+    if ((rc = dev_alloc(g, &g->d_hub64, (size_t)ROUND_SLOTS * 2, "counters")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_seeds, (size_t)MM_MAX_ROUNDS + 1, "seeds")) != MM_OK) return fail(rc);
     if (hipHostMalloc((void **)&g->h_ctr, sizeof(int32_t) * ROUND_SLOTS * CTR_STRIDE) != hipSuccess ||
         hipHostMalloc((void **)&g->h_ctr64, sizeof(unsigned long long) * ROUND_SLOTS * CTR64_STRIDE) != hipSuccess)
         return fail(mm::set_error(MM_ERR_NOMEM, "pinned host allocation failed"));
-    if (n > 0 && hipMemcpy(g->d_ranges, ranges, sizeof(int2) * nn, hipMemcpyHostToDevice) != hipSuccess)
+    if ((rc = dev_alloc(g, &g->d_loc, nn, "original ids")) != MM_OK) return fail(rc);
+    // Device copy of the CSR, rows in the internal order: each row keeps its
+    // entries in their original order (CSR position decides ties), neighbour
+    // ids are renumbered, weights are interleaved with the ids.
+    g->relabelled = relabel != 0 && n > 1;
+    std::vector<int32_t> order;
+    if (g->relabelled) order = bfs_order(n, ranges, nbrs);
+    std::vector<int32_t> inv;
+    if (g->relabelled) {
+        inv.resize(nn);
+        for (int32_t i = 0; i < n; ++i) inv[(size_t)order[(size_t)i]] = i;
+    }
+    auto old_of = [&](int64_t i) -> int32_t { return g->relabelled ? order[(size_t)i] : (int32_t)i; };
+    std::vector<int32_t> h_ranges(2 * nn), h_loc(nn);
+    int64_t off = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const int32_t o = old_of(i);
+        const int32_t deg = ranges[2 * (size_t)o + 1] - ranges[2 * (size_t)o];
+        h_ranges[2 * (size_t)i] = (int32_t)off;
+        h_ranges[2 * (size_t)i + 1] = (int32_t)(off + deg);
+        h_loc[(size_t)i] = o - (vbase_old ? vbase_old[(size_t)o] : 0);
+        off += deg;
+    }
+    std::vector<int32_t> h_nbrs;
+    std::vector<int2> h_adj;
+    if (g->weighted) h_adj.resize((size_t)nnz);
+    else h_nbrs.resize((size_t)nnz);
+    mm::parallel_for(n, [&](int64_t b, int64_t e, unsigned) {
+        for (int64_t i = b; i < e; ++i) {
+            const int32_t o = old_of(i);
+            const int32_t src = ranges[2 * (size_t)o];
+            const int32_t deg = ranges[2 * (size_t)o + 1] - src;
+            const int32_t dst = h_ranges[2 * (size_t)i];
+            for (int32_t k = 0; k < deg; ++k) {
+                const int32_t u = nbrs[src + k];
+                const int32_t un = g->relabelled ? inv[(size_t)u] : u;
+                if (g->weighted) {
+                    int32_t wb;
+                    std::memcpy(&wb, w + src + k, sizeof(wb));
+                    h_adj[(size_t)dst + k] = make_int2(un, wb);
+                } else {
+                    h_nbrs[(size_t)dst + k] = un;
+                }
+            }
+        }
+    });
+    if (n > 0 && (hipMemcpy(g->d_ranges, h_ranges.data(), sizeof(int2) * nn, hipMemcpyHostToDevice) != hipSuccess ||
+                  hipMemcpy(g->d_loc, h_loc.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess))
         return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
     if (nnz > 0) {
-        if (g->weighted) {
-            int32_t *tn = nullptr;
-            float *tw = nullptr;
-            if (hipMalloc((void **)&tn, sizeof(int32_t) * (size_t)nnz) != hipSuccess ||
-                hipMalloc((void **)&tw, sizeof(float) * (size_t)nnz) != hipSuccess) {
-                if (tn) hipFree(tn);
-                return fail(mm::set_error(MM_ERR_NOMEM, "Not enough memory on device to stage graph weights"));
-            }
-            hipError_t e1 = hipMemcpy(tn, nbrs, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice);
-            hipError_t e2 = hipMemcpy(tw, w, sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice);
-            if (e1 == hipSuccess && e2 == hipSuccess) {
-                const int blocks = (int)std::min<int64_t>((nnz + 255) / 256, (int64_t)g->num_cu * 16);
-                hipLaunchKernelGGL(k_interleave, dim3(blocks), dim3(256), 0, g->stream, g->d_adj, tn, tw, nnz);
-                e1 = hipStreamSynchronize(g->stream);
-            }
-            hipFree(tn);
-            hipFree(tw);
-            if (e1 != hipSuccess || e2 != hipSuccess)
-                return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph weights to device"));
-        } else if (hipMemcpy(g->d_nbrs, nbrs, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess) {
+        const hipError_t e = g->weighted
+                                 ? hipMemcpy(g->d_adj, h_adj.data(), sizeof(int2) * (size_t)nnz, hipMemcpyHostToDevice)
+                                 : hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz,
+                                             hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+    }
+    if (g->relabelled) {
+        if ((rc = dev_alloc(g, &g->d_old_of, nn, "vertex order")) != MM_OK) return fail(rc);
+        if ((rc = dev_alloc(g, &g->d_match_out, nn, "match")) != MM_OK) return fail(rc);
+        if (hipMemcpy(g->d_old_of, order.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess)
             return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
-        }
     }
     *out = g;
     return MM_OK;
 }
 
-int pick_lanes(const mm_graph *g, int requested)
-{
-    if (requested > 0) return requested;
-    int G = 1;
-    while (G < 32 && (double)G * 1.5 < g->mean_degree) G <<= 1;
-    return G;
-}
+// 0 selects the entry-parallel tile kernels; a power of two selects G lanes per row.
+int pick_lanes(const mm_graph *, int requested) { return requested > 0 ? requested : 0; }
 
 int64_t hubs_above(const mm_graph *g, int32_t thr)
 {
@@ -725,20 +1260,20 @@ int64_t hubs_above(const mm_graph *g, int32_t thr)
 
 struct RunCfg {
     bool W, A8, PUSH;
-    int G, block, hub_deg, grid_scan, grid_vertex, hub_blocks;
+    int G, block, hub_deg, nb, bcap, hub_blocks;
 };
 
 template <bool W, bool A8, bool PUSH, int G>
 void launch_request_t(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_request<W, A8, PUSH, G>), dim3(c.grid_scan + c.hub_blocks), dim3(c.block), 0, s, d, r,
+    hipLaunchKernelGGL((k_request<W, A8, PUSH, G>), dim3(c.nb + c.hub_blocks), dim3(c.block), 0, s, d, r,
                        c.hub_blocks);
 }
 
 template <bool W, bool A8, int G>
 void launch_respond_t(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_respond<W, A8, G>), dim3(c.grid_scan + c.hub_blocks), dim3(c.block), 0, s, d, r,
+    hipLaunchKernelGGL((k_respond<W, A8, G>), dim3(c.nb + c.hub_blocks), dim3(c.block), 0, s, d, r,
                        c.hub_blocks);
 }
 
@@ -770,8 +1305,25 @@ void launch_respond_g(const RunCfg &c, const Dev &d, int r, hipStream_t s)
     }
 }
 
+template <bool W, bool A8, bool PUSH>
+void launch_request_tile(const RunCfg &c, const Dev &d, int r, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_request_tile<W, A8, PUSH>), dim3(c.nb + c.hub_blocks), dim3(c.block),
+                       tile_smem_bytes(c.block), s, d, r, c.hub_blocks);
+}
+
 void launch_request(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
+    if (c.G == 0) {
+        if (c.W) {
+            if (c.A8) { if (c.PUSH) launch_request_tile<true, true, true>(c, d, r, s); else launch_request_tile<true, true, false>(c, d, r, s); }
+            else { if (c.PUSH) launch_request_tile<true, false, true>(c, d, r, s); else launch_request_tile<true, false, false>(c, d, r, s); }
+        } else {
+            if (c.A8) { if (c.PUSH) launch_request_tile<false, true, true>(c, d, r, s); else launch_request_tile<false, true, false>(c, d, r, s); }
+            else { if (c.PUSH) launch_request_tile<false, false, true>(c, d, r, s); else launch_request_tile<false, false, false>(c, d, r, s); }
+        }
+        return;
+    }
     if (c.W) {
         if (c.A8) { if (c.PUSH) launch_request_g<true, true, true>(c, d, r, s); else launch_request_g<true, true, false>(c, d, r, s); }
         else { if (c.PUSH) launch_request_g<true, false, true>(c, d, r, s); else launch_request_g<true, false, false>(c, d, r, s); }
@@ -781,8 +1333,20 @@ void launch_request(const RunCfg &c, const Dev &d, int r, hipStream_t s)
     }
 }
 
+template <bool W, bool A8>
+void launch_respond_tile(const RunCfg &c, const Dev &d, int r, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_respond_tile<W, A8>), dim3(c.nb + c.hub_blocks), dim3(c.block), tile_smem_bytes(c.block), s,
+                       d, r, c.hub_blocks);
+}
+
 void launch_respond(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
+    if (c.G == 0) {
+        if (c.W) { if (c.A8) launch_respond_tile<true, true>(c, d, r, s); else launch_respond_tile<true, false>(c, d, r, s); }
+        else { if (c.A8) launch_respond_tile<false, true>(c, d, r, s); else launch_respond_tile<false, false>(c, d, r, s); }
+        return;
+    }
     if (c.W) { if (c.A8) launch_respond_g<true, true>(c, d, r, s); else launch_respond_g<true, false>(c, d, r, s); }
     else { if (c.A8) launch_respond_g<false, true>(c, d, r, s); else launch_respond_g<false, false>(c, d, r, s); }
 }
@@ -799,7 +1363,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     const int block = o->block_size > 0 ? o->block_size : 256;
     if (block < 64 || block > 1024 || block % 64) return mm::set_error(MM_ERR_INVALID, "mm_match: bad block_size");
     const int G = pick_lanes(g, o->lanes_per_vertex);
-    if (G < 1 || G > 64 || (G & (G - 1))) return mm::set_error(MM_ERR_INVALID, "mm_match: bad lanes_per_vertex");
+    if (G < 0 || G > 64 || (G & (G - 1))) return mm::set_error(MM_ERR_INVALID, "mm_match: bad lanes_per_vertex");
     RunCfg c;
     c.W = o->kind == MM_WEIGHTED;
     c.A8 = g->weighted != 0;
@@ -812,11 +1376,10 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     c.block = block;
     c.hub_deg = o->hub_degree > 0 ? o->hub_degree : 256;
     const int blocks_per_cu = std::max(1, 2048 / block);
-    const int64_t groups_per_block = block / G;
-    c.grid_scan = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)g->n + groups_per_block - 1) / groups_per_block,
-                                                               (int64_t)g->num_cu * blocks_per_cu));
-    c.grid_vertex = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)g->n + block - 1) / block,
-                                                                 (int64_t)g->num_cu * blocks_per_cu));
+    // One bucket per block of the per-bucket kernels, about 2 * block ids each.
+    c.nb = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)g->n + 2 * block - 1) / (2 * block),
+                                                        std::min<int64_t>(g->nb_max, (int64_t)g->num_cu * blocks_per_cu)));
+    c.bcap = (int)(((int64_t)g->n + c.nb - 1) / c.nb);
     c.hub_blocks = (int)std::min<int64_t>(hubs_above(g, c.hub_deg), (int64_t)g->num_cu * blocks_per_cu);
 
     if (hipSetDevice(g->device) != hipSuccess) return mm::set_error(MM_ERR_HIP, "hipSetDevice failed");
@@ -829,17 +1392,15 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
                                   hipMemcpyHostToDevice, g->stream),
                    "Unable to set seeds");
     }
-    if (g->d_glast && !g->h_glast_init.empty())
-        MM_HIP_TRY(hipMemcpyAsync(g->d_glast, g->h_glast_init.data(), sizeof(int32_t) * g->h_glast_init.size(),
-                                  hipMemcpyHostToDevice, g->stream),
-                   "Unable to reset batch state");
 
     Dev d;
     d.n = g->n;
-    d.cap = g->n;
+    d.cap = (int32_t)g->list_cap;
     d.hub_deg = c.hub_deg;
     d.barrier = o->barrier;
     d.count_nnz = o->profile ? 1 : 0;
+    d.nb = c.nb;
+    d.bcap = c.bcap;
     d.ranges = g->d_ranges;
     d.nbrs = g->d_nbrs;
     d.adj = g->d_adj;
@@ -849,13 +1410,16 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.resp = g->d_resp;
     d.lists = g->d_lists;
     d.ctr = g->d_ctr;
+    d.ctrb = g->d_ctrb;
     d.ctr64 = g->d_ctr64;
+    d.ctrb64 = g->d_ctrb64;
+    d.hub64 = g->d_hub64;
     d.seeds = g->d_seeds;
-    d.vbase = g->d_vbase;
-    d.vgid = g->d_vgid;
-    d.glast = g->d_glast;
+    d.loc = g->d_loc;
+    d.vround = g->d_vround;
 
     const bool prof = o->profile != 0;
+    const bool use_graph = !prof && o->graph_mode == 0;
     size_t ev_used = 0;
     float kms[MM_KERNEL_CLASSES] = {0};
     int klaunch[MM_KERNEL_CLASSES] = {0};
@@ -886,49 +1450,107 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
         if ((rc = ev_end()) != MM_OK) return rc;         \
     } while (0)
 
-    MM_LAUNCH(K_CLEAR, hipLaunchKernelGGL(k_clear, dim3(8), dim3(256), 0, g->stream, g->d_ctr, g->d_ctr64,
-                                          ROUND_SLOTS * CTR_STRIDE, ROUND_SLOTS * CTR64_STRIDE));
-    MM_HIP_TRY(hipEventRecord(g->ev_t1, g->stream), "hipEventRecord");
-    if (g->n > 0) {
-        if (c.PUSH) MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select<true>, dim3(c.grid_vertex), dim3(block), 0, g->stream, d));
-        else MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select<false>, dim3(c.grid_vertex), dim3(block), 0, g->stream, d));
-    }
-    int rounds = -1;
-    const int chunk = 4;
     // A maximal run may stop early; it can only go as far as the seeds given.
     const int R_run = maximal ? std::min(R_max, o->n_seeds) : R_max;
-    int r = 1;
-    while (r <= R_run && g->n > 0) {
-        const int end = std::min(R_run, r + chunk - 1);
-        for (int rr = r; rr <= end; ++rr) {
+    // Fixed runs are one launch sequence; maximal runs check the round
+    // counters after every chunk of rounds (rounds past convergence are no-ops).
+    const int chunk = maximal ? 4 : R_run;
+    auto enq_reduce = [&](int r0, int r1) -> int32_t {
+        MM_LAUNCH(K_CLEAR, hipLaunchKernelGGL(k_reduce_counts, dim3(r1 - r0 + 1), dim3(256), 0, g->stream, d, r0,
+                                              d.count_nnz));
+        return MM_OK;
+    };
+    // Enqueues rounds [r0, r1] (and the clear + round-1 select when r0 == 1);
+    // maximal runs also total the counters of rounds [r0, r1 + 1].
+    auto enqueue = [&](int r0, int r1) -> int32_t {
+        if (r0 == 1) {
+            MM_LAUNCH(K_CLEAR, hipLaunchKernelGGL(k_clear, dim3(8), dim3(256), 0, g->stream, g->d_ctr, g->d_hub64,
+                                                  ROUND_SLOTS * CTR_STRIDE, ROUND_SLOTS * 2));
+            if (c.PUSH) MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select<true>, dim3(c.nb), dim3(block), 0, g->stream, d));
+            else MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select<false>, dim3(c.nb), dim3(block), 0, g->stream, d));
+        }
+        for (int rr = r0; rr <= r1; ++rr) {
             const int do_select = rr < R_run;
             MM_LAUNCH(K_REQUEST, launch_request(c, d, rr, g->stream));
             if (!c.PUSH) MM_LAUNCH(K_RESPOND, launch_respond(c, d, rr, g->stream));
-            if (c.PUSH) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL(k_match_select<true>, dim3(c.grid_vertex), dim3(block), 0, g->stream, d, rr, do_select));
-            else MM_LAUNCH(K_MATCH, hipLaunchKernelGGL(k_match_select<false>, dim3(c.grid_vertex), dim3(block), 0, g->stream, d, rr, do_select));
+            if (c.PUSH) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL(k_match_select<true>, dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
+            else MM_LAUNCH(K_MATCH, hipLaunchKernelGGL(k_match_select<false>, dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
         }
-        MM_HIP_TRY(hipGetLastError(), "A HIP error occurred during the matching process");
-        if (maximal) {
-            const int k_hi = std::min(end + 1, R_run);
-            MM_HIP_TRY(hipMemcpyAsync(g->h_ctr + (size_t)r * CTR_STRIDE, g->d_ctr + (size_t)r * CTR_STRIDE,
-                                      sizeof(int32_t) * (size_t)(k_hi - r + 1) * CTR_STRIDE, hipMemcpyDeviceToHost,
-                                      g->stream),
-                       "Unable to read round counters");
-            MM_HIP_TRY(hipStreamSynchronize(g->stream), "A HIP error occurred during the matching process");
-            for (int k = r; k <= k_hi; ++k) {
-                const int32_t *q = g->h_ctr + (size_t)k * CTR_STRIDE;
-                if ((int64_t)q[0] + q[1] + q[2] + q[3] == 0) { rounds = k; break; }
+        if (maximal) return enq_reduce(r0, std::min(r1 + 1, R_run));
+        return MM_OK;
+    };
+    // Replays the captured launch sequence of rounds [r0, r1], capturing it on first use.
+    ExecKey key;
+    std::memset(&key, 0, sizeof(key));
+    key.R_run = R_run; key.W = c.W; key.A8 = c.A8; key.PUSH = c.PUSH; key.G = c.G; key.block = c.block;
+    key.hub_deg = c.hub_deg; key.nb = c.nb; key.bcap = c.bcap; key.maximal = maximal;
+    key.hub_blocks = c.hub_blocks; key.barrier = o->barrier; key.count_nnz = d.count_nnz;
+    auto run_chunk = [&](int r0, int r1) -> int32_t {
+        if (!use_graph) return enqueue(r0, r1);
+        key.r0 = r0;
+        key.r1 = r1;
+        for (const CachedExec &ce : g->execs)
+            if (std::memcmp(&ce.key, &key, sizeof(key)) == 0) {
+                MM_HIP_TRY(hipGraphLaunch(ce.exec, g->stream), "hipGraphLaunch");
+                return MM_OK;
             }
-            if (rounds > 0) break;
+        hipGraph_t graph = nullptr;
+        MM_HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        const int32_t erc = enqueue(r0, r1);
+        const hipError_t ec = hipStreamEndCapture(g->stream, &graph);
+        if (erc != MM_OK) { if (graph) hipGraphDestroy(graph); return erc; }
+        MM_HIP_TRY(ec, "hipStreamEndCapture");
+        CachedExec ce;
+        ce.key = key;
+        const hipError_t ei = hipGraphInstantiate(&ce.exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        MM_HIP_TRY(ei, "hipGraphInstantiate");
+        if (g->execs.size() >= 256) {
+            hipGraphExecDestroy(g->execs.front().exec);
+            g->execs.erase(g->execs.begin());
         }
-        r = end + 1;
+        g->execs.push_back(ce);
+        MM_HIP_TRY(hipGraphLaunch(ce.exec, g->stream), "hipGraphLaunch");
+        return MM_OK;
+    };
+
+    MM_HIP_TRY(hipEventRecord(g->ev_t1, g->stream), "hipEventRecord");
+    int rounds = -1;
+    int r = 1;
+    if (g->n > 0) {
+        while (r <= R_run) {
+            const int end = std::min(R_run, r + chunk - 1);
+            if ((rc = run_chunk(r, end)) != MM_OK) return rc;
+            MM_HIP_TRY(hipGetLastError(), "A HIP error occurred during the matching process");
+            if (maximal) {
+                const int k_hi = std::min(end + 1, R_run);
+                MM_HIP_TRY(hipMemcpyAsync(g->h_ctr + (size_t)r * CTR_STRIDE, g->d_ctr + (size_t)r * CTR_STRIDE,
+                                          sizeof(int32_t) * (size_t)(k_hi - r + 1) * CTR_STRIDE,
+                                          hipMemcpyDeviceToHost, g->stream),
+                           "Unable to read round counters");
+                MM_HIP_TRY(hipStreamSynchronize(g->stream), "A HIP error occurred during the matching process");
+                for (int k = r; k <= k_hi; ++k) {
+                    const int32_t *q = g->h_ctr + (size_t)k * CTR_STRIDE;
+                    if ((int64_t)q[0] + q[1] + q[2] + q[3] == 0) { rounds = k; break; }
+                }
+                if (rounds > 0) break;
+            }
+            r = end + 1;
+        }
     }
     if (maximal && rounds < 0 && R_run < R_max && g->n > 0)
         return mm::set_error(MM_ERR_SEEDS, "mm_match: ran out of seeds before the matching converged");
     if (g->n == 0) rounds = maximal ? 1 : R_max;
     if (rounds < 0) rounds = R_max;
+    // The result in the caller's numbering is part of the matching.
+    if (g->relabelled && g->n > 0)
+        hipLaunchKernelGGL(k_unpermute, dim3(std::min<int64_t>(((int64_t)g->n + 255) / 256, (int64_t)g->num_cu * 8)),
+                           dim3(256), 0, g->stream, g->d_match, g->d_old_of, g->d_match_out, g->n);
     MM_HIP_TRY(hipEventRecord(g->ev_t2, g->stream), "hipEventRecord");
-    if (stats) {
+    // Totals of every executed round (a maximal run's last, empty round has none).
+    const int r_stats = g->n > 0 ? std::min(rounds, R_run) : 0;
+    if (stats && r_stats > 0) {
+        if ((rc = enq_reduce(1, r_stats)) != MM_OK) return rc;
         MM_HIP_TRY(hipMemcpyAsync(g->h_ctr, g->d_ctr, sizeof(int32_t) * ROUND_SLOTS * CTR_STRIDE,
                                   hipMemcpyDeviceToHost, g->stream),
                    "Unable to read round counters");
@@ -936,17 +1558,18 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
                                   hipMemcpyDeviceToHost, g->stream),
                    "Unable to read round counters");
     }
-    if (graph_rounds && g->d_glast) {
+    if (graph_rounds && g->d_vround) {
         graph_rounds->assign((size_t)g->n_graphs, 0);
-        MM_HIP_TRY(hipMemcpyAsync(graph_rounds->data(), g->d_glast, sizeof(int32_t) * (size_t)g->n_graphs,
-                                  hipMemcpyDeviceToHost, g->stream),
-                   "Unable to read batch rounds");
+        if (g->n_graphs > 0) {
+            hipLaunchKernelGGL(k_graph_rounds, dim3((g->n_graphs + 3) / 4), dim3(256), 0, g->stream, g->d_vround,
+                               g->d_voff, g->n_graphs, R_run, R_max, maximal, g->d_grounds);
+            MM_HIP_TRY(hipMemcpyAsync(graph_rounds->data(), g->d_grounds, sizeof(int32_t) * (size_t)g->n_graphs,
+                                      hipMemcpyDeviceToHost, g->stream),
+                       "Unable to read batch rounds");
+        }
     }
     MM_HIP_TRY(hipStreamSynchronize(g->stream), "A HIP error occurred during the matching process");
     MM_HIP_TRY(hipGetLastError(), "A HIP error occurred during the matching process");
-    if (graph_rounds && g->d_glast) {
-        for (auto &x : *graph_rounds) x = maximal ? std::min(x + 1, R_max) : R_max;
-    }
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->rounds = rounds;
@@ -956,16 +1579,18 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
         hipEventElapsedTime(&ms, g->ev_t1, g->ev_t2);
         stats->match_ms = ms;
         int64_t cum = 0;
-        for (int k = 1; k <= rounds && k <= MM_MAX_ROUNDS; ++k) {
+        for (int k = 1; k <= r_stats && k <= MM_MAX_ROUNDS; ++k) {
             const int32_t *q = g->h_ctr + (size_t)k * CTR_STRIDE;
             const unsigned long long *q64 = g->h_ctr64 + (size_t)k * CTR64_STRIDE;
-            cum += q[4];
-            stats->active[k - 1] = (int64_t)q[0] + q[1] + q[2] + q[3];
+            const int64_t active = (int64_t)q[0] + q[1] + q[2] + q[3];
+            if (active > 0) cum += q[4];
+            stats->active[k - 1] = active;
             stats->blue[k - 1] = (int64_t)q[0] + q[1];
             stats->matched[k - 1] = cum;
-            stats->active_nnz[k - 1] = (int64_t)q64[0];
-            stats->blue_nnz[k - 1] = (int64_t)q64[1];
+            stats->active_nnz[k - 1] = active > 0 ? (int64_t)q64[0] : 0;
+            stats->blue_nnz[k - 1] = active > 0 ? (int64_t)q64[1] : 0;
         }
+        for (int k = r_stats + 1; k <= rounds && k <= MM_MAX_ROUNDS; ++k) stats->matched[k - 1] = cum;
         for (int k = 0; k < MM_KERNEL_CLASSES; ++k) stats->kernel_launches[k] = klaunch[k];
         if (prof) {
             for (size_t e = 0; e < ev_class.size(); ++e) {
@@ -994,7 +1619,15 @@ int32_t mm_device_count(void)
 int32_t mm_graph_create(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *neighbours,
                         const float *weights, mm_graph **out)
 {
-    return create_graph(device, n, nnz, ranges, neighbours, weights, out);
+    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, 1, out);
+}
+
+int32_t mm_graph_create_ex(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *neighbours,
+                           const float *weights, uint32_t flags, mm_graph **out)
+{
+    if (flags & ~(uint32_t)MM_CREATE_KEEP_ORDER) return mm::set_error(MM_ERR_INVALID, "mm_graph_create_ex: bad flags");
+    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr,
+                        (flags & MM_CREATE_KEEP_ORDER) ? 0 : 1, out);
 }
 
 int32_t mm_graph_destroy(mm_graph *g)
@@ -1030,7 +1663,8 @@ int32_t mm_graph_copy_match(mm_graph *g, int32_t *match_out)
     if (!g || !match_out) return mm::set_error(MM_ERR_INVALID, "mm_graph_copy_match: NULL argument");
     if (g->n == 0) return MM_OK;
     if (hipSetDevice(g->device) != hipSuccess) return mm::set_error(MM_ERR_HIP, "hipSetDevice failed");
-    MM_HIP_TRY(hipMemcpyAsync(match_out, g->d_match, sizeof(int32_t) * (size_t)g->n, hipMemcpyDeviceToHost, g->stream),
+    MM_HIP_TRY(hipMemcpyAsync(match_out, g->relabelled ? g->d_match_out : g->d_match, sizeof(int32_t) * (size_t)g->n,
+                              hipMemcpyDeviceToHost, g->stream),
                "Unable to retrieve data");
     MM_HIP_TRY(hipStreamSynchronize(g->stream), "Unable to retrieve data");
     return MM_OK;
@@ -1052,7 +1686,7 @@ int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_
             return mm::set_error(MM_ERR_INVALID, "mm_batch_create: offsets must be non-decreasing");
     // Rebase every graph into one global CSR.
     std::vector<int32_t> granges((size_t)total_n * 2), gnbrs((size_t)total_nnz);
-    std::vector<int32_t> vbase((size_t)total_n), vgid((size_t)total_n);
+    std::vector<int32_t> vbase((size_t)total_n);
     int bad = 0;
     mm::parallel_for(n_graphs, [&](int64_t b, int64_t e, unsigned) {
         for (int64_t k = b; k < e; ++k) {
@@ -1065,7 +1699,6 @@ int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_
                 granges[2 * (size_t)v] = (int32_t)(e0 + lo);
                 granges[2 * (size_t)v + 1] = (int32_t)(e0 + hi);
                 vbase[(size_t)v] = v0;
-                vgid[(size_t)v] = (int32_t)k;
             }
             for (int64_t j = e0; j < e1; ++j) {
                 const int32_t u = neighbours[j];
@@ -1076,20 +1709,18 @@ int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_
     });
     if (bad) return mm::set_error(MM_ERR_INVALID, "mm_batch_create: malformed per-graph CSR");
     mm_graph *g = nullptr;
-    int32_t rc = create_graph(device, (int32_t)total_n, total_nnz, granges.data(), gnbrs.data(), weights, &g);
+    int32_t rc = create_graph(device, (int32_t)total_n, total_nnz, granges.data(), gnbrs.data(), weights,
+                              vbase.data(), 1, &g);
     if (rc != MM_OK) return rc;
     g->n_graphs = n_graphs;
-    if ((rc = dev_alloc(g, &g->d_vbase, (size_t)total_n, "batch bases")) != MM_OK ||
-        (rc = dev_alloc(g, &g->d_vgid, (size_t)total_n, "batch ids")) != MM_OK ||
-        (rc = dev_alloc(g, &g->d_glast, (size_t)n_graphs, "batch rounds")) != MM_OK) {
+    if ((rc = dev_alloc(g, &g->d_voff, (size_t)n_graphs + 1, "batch offsets")) != MM_OK ||
+        (rc = dev_alloc(g, &g->d_vround, (size_t)total_n, "batch rounds")) != MM_OK ||
+        (rc = dev_alloc(g, &g->d_grounds, (size_t)n_graphs, "batch rounds")) != MM_OK) {
         release(g);
         return rc;
     }
-    g->h_glast_init.resize((size_t)n_graphs);
-    for (int32_t k = 0; k < n_graphs; ++k) g->h_glast_init[(size_t)k] = vertex_offsets[k + 1] > vertex_offsets[k] ? 1 : 0;
-    if (total_n > 0 &&
-        (hipMemcpy(g->d_vbase, vbase.data(), sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice) != hipSuccess ||
-         hipMemcpy(g->d_vgid, vgid.data(), sizeof(int32_t) * (size_t)total_n, hipMemcpyHostToDevice) != hipSuccess)) {
+    if (hipMemcpy(g->d_voff, vertex_offsets, sizeof(int32_t) * ((size_t)n_graphs + 1), hipMemcpyHostToDevice) !=
+        hipSuccess) {
         release(g);
         return mm::set_error(MM_ERR_HIP, "mm_batch_create: upload failed");
     }

@@ -29,8 +29,9 @@ MM_NR_MATCH_ROUNDS = 20
 MM_NR_MAX_MATCH_ROUNDS = 255
 MM_MAX_ROUNDS = 255
 MM_DEFAULT_BARRIER = 0x88B81733
+MM_CREATE_KEEP_ORDER = 1
 MM_KERNEL_CLASSES = 5
-KERNEL_CLASS_NAMES = ("init_select", "request", "respond", "match_select", "clear")
+KERNEL_CLASS_NAMES = ("init_select", "request", "respond", "match_select", "counters")
 
 # Reference matcher type codes of the GPU path (bin/match.cpp:145-148).
 TYPE_GPU_RANDOM, TYPE_GPU_RANDOM_MAXIMAL, TYPE_GPU_WEIGHTED, TYPE_GPU_WEIGHTED_MAXIMAL = 6, 7, 8, 9
@@ -56,6 +57,7 @@ class _Opts(ctypes.Structure):
         ("hub_degree", ctypes.c_int32),
         ("respond", ctypes.c_int32),
         ("profile", ctypes.c_int32),
+        ("graph_mode", ctypes.c_int32),
     ]
 
 
@@ -107,6 +109,7 @@ SYMBOLS = {
     "mm_last_error": (ctypes.c_char_p, []),
     "mm_device_count": (_I32, []),
     "mm_graph_create": (_I32, [_I32, _I32, _I64, _P, _P, _P, ctypes.POINTER(_P)]),
+    "mm_graph_create_ex": (_I32, [_I32, _I32, _I64, _P, _P, _P, ctypes.c_uint32, ctypes.POINTER(_P)]),
     "mm_graph_destroy": (_I32, [_P]),
     "mm_graph_get_info": (_I32, [_P, ctypes.POINTER(_Info)]),
     "mm_match": (_I32, [_P, ctypes.POINTER(_Opts), _P, ctypes.POINTER(_Stats)]),
@@ -270,7 +273,8 @@ class MatchResult:
     kernel_launches: dict = field(default_factory=dict)
 
 
-def _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond, profile):
+def _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond, profile,
+               graph_mode=0):
     seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
     o = _Opts()
     o.kind = kind
@@ -284,6 +288,7 @@ def _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub
     o.hub_degree = hub_degree
     o.respond = respond
     o.profile = int(profile)
+    o.graph_mode = int(graph_mode)
     return o, seeds
 
 
@@ -304,14 +309,16 @@ def _result(st: _Stats, match) -> MatchResult:
 class Graph:
     """Device-resident CSR graph (mm_graph)."""
 
-    def __init__(self, csr: CSR, device: int = 0):
+    def __init__(self, csr: CSR, device: int = 0, keep_order: bool = False):
         self.csr = csr
         self.n, self.nnz = csr.n, csr.nnz
         h = ctypes.c_void_p()
         ranges = np.ascontiguousarray(csr.ranges, dtype=np.int32)
         nb = np.ascontiguousarray(csr.neighbours, dtype=np.int32)
         w = None if csr.weights is None else np.ascontiguousarray(csr.weights, dtype=np.float32)
-        _check(lib().mm_graph_create(device, self.n, self.nnz, _ptr(ranges), _ptr(nb), _ptr(w), ctypes.byref(h)))
+        flags = MM_CREATE_KEEP_ORDER if keep_order else 0
+        _check(lib().mm_graph_create_ex(device, self.n, self.nnz, _ptr(ranges), _ptr(nb), _ptr(w), flags,
+                                        ctypes.byref(h)))
         self._h = h
 
     def info(self) -> dict:
@@ -322,13 +329,13 @@ class Graph:
     def match(self, kind: int = MM_WEIGHTED, maximal: bool = True, seeds=None, max_rounds: int = 0,
               barrier: int = MM_DEFAULT_BARRIER, block_size: int = 0, lanes: int = 0, hub_degree: int = 0,
               respond: int = MM_RESPOND_AUTO, profile: bool = False, fetch: bool = True,
-              stats: bool = True) -> MatchResult | None:
+              stats: bool = True, graph_mode: int = 0) -> MatchResult | None:
         """Runs one matching.  fetch=False leaves the match array on the device;
         stats=False skips the counter read-back and returns None."""
         if seeds is None:
             seeds = reference_seeds(MM_MAX_ROUNDS)
         o, keep = _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond,
-                             profile)
+                             profile, graph_mode)
         st = _Stats() if stats else None
         out = np.empty(self.n, np.int32) if fetch else None
         _check(lib().mm_match(self._h, ctypes.byref(o), _ptr(out), ctypes.byref(st) if stats else None))
@@ -381,11 +388,11 @@ class Batch:
 
     def match(self, kind: int = MM_WEIGHTED, maximal: bool = True, seeds=None, max_rounds: int = 0,
               barrier: int = MM_DEFAULT_BARRIER, block_size: int = 0, lanes: int = 0, hub_degree: int = 0,
-              respond: int = MM_RESPOND_AUTO, profile: bool = False, fetch: bool = True):
+              respond: int = MM_RESPOND_AUTO, profile: bool = False, fetch: bool = True, graph_mode: int = 0):
         if seeds is None:
             seeds = reference_seeds(MM_MAX_ROUNDS)
         o, keep = _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond,
-                             profile)
+                             profile, graph_mode)
         st = _Stats()
         out = np.empty(self.total_n, np.int32) if fetch else None
         rounds = np.zeros(len(self.graphs), np.int32)

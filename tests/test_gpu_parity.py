@@ -51,16 +51,18 @@ def test_golden_tiny_cases_all_types_and_tiles():
     for idx, c in enumerate(data["cases"]):
         say("case", idx, c["spec"], "weighted", c["weighted"], "maximal", c["maximal"])
         csr = case_csr(c)
-        g = Graph(csr)
-        modes = [MM_RESPOND_PULL] + ([MM_RESPOND_PUSH] if g.info()["push_ok"] else [])
-        for respond in modes:
-            for lanes, block, hub in ((1, 64, 2), (4, 256, 256), (16, 256, 3), (64, 128, 1)):
-                res = g.match(kind=MM_WEIGHTED if c["weighted"] else MM_RANDOM, maximal=c["maximal"], seeds=seeds,
-                              max_rounds=c["max_rounds"], barrier=c["barrier"], lanes=lanes, block_size=block,
-                              hub_degree=hub, respond=respond)
-                assert res.rounds == c["rounds"], (idx, respond, lanes)
-                assert res.match.tolist() == c["match"], (idx, respond, lanes, block, hub)
-        g.close()
+        for keep in (False, True):
+            g = Graph(csr, keep_order=keep)
+            modes = [MM_RESPOND_PULL] + ([MM_RESPOND_PUSH] if g.info()["push_ok"] else [])
+            for respond in modes:
+                for lanes, block, hub in ((0, 256, 256), (0, 64, 3), (0, 1024, 1), (1, 64, 2), (4, 256, 256),
+                                          (16, 256, 3), (64, 128, 1)):
+                    res = g.match(kind=MM_WEIGHTED if c["weighted"] else MM_RANDOM, maximal=c["maximal"],
+                                  seeds=seeds, max_rounds=c["max_rounds"], barrier=c["barrier"], lanes=lanes,
+                                  block_size=block, hub_degree=hub, respond=respond)
+                    assert res.rounds == c["rounds"], (idx, keep, respond, lanes)
+                    assert res.match.tolist() == c["match"], (idx, keep, respond, lanes, block, hub)
+            g.close()
 
 
 @pytest.mark.parametrize("maximal", [False, True])
@@ -73,11 +75,11 @@ def test_rgg16_random_bitexact(maximal):
     assert oracle.test_matching(csr, res.match)[0]
 
 
-@pytest.mark.parametrize("maximal", [False, True])
-def test_rgg18_weighted_bitexact(maximal):
-    say("rgg18 weighted maximal", maximal)
+@pytest.mark.parametrize("maximal,keep", [(False, False), (True, False), (True, True)])
+def test_rgg18_weighted_bitexact(maximal, keep):
+    say("rgg18 weighted maximal", maximal, "keep_order", keep)
     csr = mmamd.gen_rgg(1 << 18, seed=777, weighted=True)
-    g = Graph(csr)
+    g = Graph(csr, keep_order=keep)
     assert g.info()["push_ok"] == 1
     for respond in (MM_RESPOND_PULL, MM_RESPOND_PUSH):
         res = check_against_oracle(csr, g, True, maximal, respond=respond)
@@ -108,14 +110,16 @@ def shuffled(csr, seed):
     return CSR(np.ascontiguousarray(ranges, dtype=np.int32), nb, csr.weights)
 
 
-def test_shuffled_rows_use_exact_pull_path():
-    say("shuffled rgg")
+@pytest.mark.parametrize("keep", [False, True])
+def test_shuffled_rows_use_exact_pull_path(keep):
+    say("shuffled rgg keep_order", keep)
     csr = shuffled(mmamd.gen_rgg(1 << 16, seed=31, weighted=True), 4)
-    g = Graph(csr)
+    g = Graph(csr, keep_order=keep)
     assert g.info()["push_ok"] == 0
     for weighted in (False, True):
-        res = check_against_oracle(csr, g, weighted, True)
-        assert res.respond == MM_RESPOND_PULL
+        for lanes in (0, 8):
+            res = check_against_oracle(csr, g, weighted, True, lanes=lanes)
+            assert res.respond == MM_RESPOND_PULL
 
 
 def test_tile_choices_do_not_change_results():
@@ -123,7 +127,7 @@ def test_tile_choices_do_not_change_results():
     csr = mmamd.gen_rgg(1 << 15, seed=8, weighted=True)
     g = Graph(csr)
     ref = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS)
-    for lanes in (1, 2, 4, 8, 16, 32, 64):
+    for lanes in (0, 1, 2, 4, 8, 16, 32, 64):
         for block in (64, 256, 1024):
             for hub in (4, 12, 256):
                 res = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS, lanes=lanes, block_size=block,
@@ -131,15 +135,16 @@ def test_tile_choices_do_not_change_results():
                 assert res.rounds == ref.rounds and np.array_equal(res.match, ref.match), (lanes, block, hub)
 
 
-@pytest.mark.parametrize("hub", [16, 256])
-def test_rmat_hub_path_bitexact(hub):
-    say("rmat scale 15 hub", hub)
+@pytest.mark.parametrize("hub,keep", [(16, False), (256, False), (16, True)])
+def test_rmat_hub_path_bitexact(hub, keep):
+    say("rmat scale 15 hub", hub, "keep_order", keep)
     csr = mmamd.gen_rmat(15, 16, seed=99, weighted=True)
-    g = Graph(csr)
+    g = Graph(csr, keep_order=keep)
     assert g.info()["max_degree"] > 256
     for weighted in (False, True):
         for respond in (MM_RESPOND_PULL, MM_RESPOND_PUSH):
-            check_against_oracle(csr, g, weighted, True, hub_degree=hub, respond=respond)
+            for lanes in (0, 32):
+                check_against_oracle(csr, g, weighted, True, hub_degree=hub, respond=respond, lanes=lanes)
 
 
 def test_batch_equals_single_graph_path():
@@ -198,3 +203,22 @@ def test_rgg22_weighted_full_size_bitexact():
     assert ok and matchable == 0
     w_gpu, size = oracle.get_weight(csr, res.match)
     assert size == res.matched[-1]
+
+
+def test_graph_replays_equal_eager_launches():
+    """Captured launch sequences are replayed many times, with eager (plain
+    stream) runs of other configurations in between; every replay must equal
+    the eager result."""
+    say("graph replay vs eager")
+    csr = mmamd.gen_rgg(1 << 16, seed=17, weighted=True)
+    g = Graph(csr)
+    eager_max = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS, graph_mode=1)
+    eager_fix = g.match(kind=MM_WEIGHTED, maximal=False, max_rounds=5, seeds=SEEDS, graph_mode=1)
+    for i in range(5):
+        rep = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS)
+        assert rep.rounds == eager_max.rounds and np.array_equal(rep.match, eager_max.match), i
+        np.testing.assert_array_equal(rep.matched, eager_max.matched)
+        mid = g.match(kind=MM_RANDOM, maximal=True, seeds=SEEDS, graph_mode=1)
+        assert oracle.test_matching(csr, mid.match)[0]
+        rep = g.match(kind=MM_WEIGHTED, maximal=False, max_rounds=5, seeds=SEEDS)
+        assert np.array_equal(rep.match, eager_fix.match), i
