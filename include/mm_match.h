@@ -49,6 +49,8 @@ extern "C" {
  * when the CSR is structurally symmetric, every row is strictly ascending and
  * w(u,v) == w(v,u) — mm_graph_create checks this and AUTO picks PUSH only
  * then. */
+#define MM_LANES_TILES (-1)
+
 #define MM_RESPOND_AUTO 0
 #define MM_RESPOND_PULL 1
 #define MM_RESPOND_PUSH 2
@@ -64,9 +66,11 @@ typedef struct mm_opts {
     const uint32_t *seeds;    /* host array, one seed per round, >= rounds that can run */
     int32_t n_seeds;
     int32_t block_size;       /* threads per block, multiple of 64 in [64, 1024]; 0 = 256 */
-    int32_t lanes_per_vertex; /* 0: entry-parallel tiles (rows staged in LDS, one row entry per thread);
-                                 1,2,4,...,64: that many lanes scan one row */
+    int32_t lanes_per_vertex; /* 1,2,4,...,64 lanes scan one row and stop at its first candidate
+                                 (0 = 4); MM_LANES_TILES: entry-parallel tiles (rows staged in LDS,
+                                 one row entry per thread, full rows) */
     int32_t hub_degree;       /* rows longer than this are scanned by a whole block; 0 = 256 */
+    int32_t entries_per_thread; /* tiles: row entries each thread keeps in flight (4, 8, 16); 0 = 4 */
     int32_t respond;          /* MM_RESPOND_* */
     int32_t profile;          /* 1: time every kernel with HIP events (slower; fills kernel_ms and *_nnz) */
     int32_t graph_mode;       /* 0: capture each launch sequence once as a hipGraph and replay it;

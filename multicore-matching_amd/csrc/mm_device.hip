@@ -59,12 +59,14 @@ struct Dev {
     int32_t bcap;         // capacity of one bucket; bucket b owns ids [b*bcap, (b+1)*bcap) in round 1
     const int2 *ranges;
     const int32_t *nbrs;  // unweighted layout
-    const int2 *adj;      // weighted layout: {neighbour, weight bits}
+    const int32_t *sid;   // weighted rows sorted for matching: neighbour ids, ~id for weights that never win
+    const float *sw;      // their weights
     int32_t *match;
     int32_t *req;
     uint8_t *st;
     unsigned long long *resp;  // [2][n], PUSH only
-    int32_t *lists;            // [2][NLISTS][cap]; classes 0 and 2 split into nb buckets of bcap
+    int4 *lists;               // [2][NLISTS][cap] entries {v, row begin, row end, loc[v]};
+                               // classes 0 and 2 split into nb buckets of bcap
     int32_t *ctr;              // [ROUND_SLOTS][CTR_STRIDE] totals; [1],[3] = hub list sizes (atomics)
     int32_t *ctrb;             // [ROUND_SLOTS][4][nb] per-bucket: blue-lo, red-lo, matched, dead
     unsigned long long *ctr64; // [ROUND_SLOTS][CTR64_STRIDE] degree sums (profile)
@@ -76,9 +78,29 @@ struct Dev {
     uint8_t *vround;           // batched: round in which the vertex stopped being active (255: never)
 };
 
-__device__ __forceinline__ int32_t *list_ptr(const Dev &d, int parity, int k)
+// Work-list entry: the vertex, its row range and its original id travel
+// together, so the scans start without dependent gathers.
+__device__ __forceinline__ int4 load_ent(const int4 *p)
+{
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i e = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(p));
+    return make_int4(e.x, e.y, e.z, e.w);
+}
+
+__device__ __forceinline__ int4 *list_ptr(const Dev &d, int parity, int k)
 {
     return d.lists + (size_t)(parity * NLISTS + k) * (size_t)d.cap;
+}
+
+// Bucket owned by this block.  Workgroups are dealt to the 8 XCDs round
+// robin (workgroup i runs on XCD i % 8), so block i takes bucket
+// (i % 8) * (nb / 8) + i / 8: every XCD owns one contiguous eighth of the
+// vertex ids, and the neighbour-state gathers of its blocks stay inside its
+// own L2.  nb is a multiple of 8.
+__device__ __forceinline__ int bucket_of_block(const Dev &d)
+{
+    const int i = blockIdx.x;
+    return (i & 7) * (d.nb >> 3) + (i >> 3);
 }
 
 __device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t s) { return (x << s) | (x >> (32u - s)); }
@@ -116,7 +138,7 @@ __device__ __forceinline__ int colour_of(uint32_t local_id, uint32_t seed, uint3
 
 // Appends value to list for every lane with pred set: one atomic per wavefront.
 // Must be reached by all lanes of the wavefront.
-__device__ __forceinline__ void wave_append(bool pred, int32_t value, int32_t *list, int32_t *counter)
+__device__ __forceinline__ void wave_append(bool pred, int4 value, int4 *list, int32_t *counter)
 {
     const unsigned long long mask = __ballot(pred);
     if (mask == 0ull) return;
@@ -128,128 +150,61 @@ __device__ __forceinline__ void wave_append(bool pred, int32_t value, int32_t *l
     if (pred) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
-// Candidate of a row scan: best weight, its CSR position and neighbour, and
-// whether any neighbour is still unmatched (the reference's "dead" test).
-struct Cand {
-    float w;
-    int32_t j;
-    int32_t u;
-    int32_t alive;
-};
-
-__device__ __forceinline__ Cand cand_none() { return Cand{-1.0f, INT_MAX, -1, 0}; }
-
-// Order of the reference's sequential scans: strict '>' from -1.0 keeps the
-// first maximum in CSR order (weighted); the first hit wins (random).
-template <bool W>
-__device__ __forceinline__ bool cand_better(const Cand &o, const Cand &m)
+// Neighbour id of row entry j; valid is false for a weight that can never
+// win (sorted weighted rows store those ids complemented).  Rows are streamed
+// once per round: non-temporal loads, so they do not evict the gathered
+// state arrays from L2.
+template <bool A8>
+__device__ __forceinline__ void load_id(const Dev &d, int32_t j, int32_t &u, bool &valid)
 {
-    if (W) return (o.w > m.w) || (o.w == m.w && o.j < m.j);
-    return o.j < m.j;
-}
-
-template <bool W>
-__device__ __forceinline__ Cand cand_shfl_reduce(Cand c, int width)
-{
-    for (int off = width >> 1; off >= 1; off >>= 1) {
-        Cand o;
-        o.w = __shfl_xor(c.w, off, width);
-        o.j = __shfl_xor(c.j, off, width);
-        o.u = __shfl_xor(c.u, off, width);
-        o.alive = __shfl_xor(c.alive, off, width);
-        const int alive = c.alive | o.alive;
-        if (cand_better<W>(o, c)) c = o;
-        c.alive = alive;
+    if (A8) {
+        const int32_t raw = __builtin_nontemporal_load(d.sid + j);
+        valid = raw >= 0;
+        u = valid ? raw : ~raw;
+    } else {
+        u = __builtin_nontemporal_load(d.nbrs + j);
+        valid = true;
     }
-    return c;
-}
-
-// Block-wide reduction; the result is valid in thread 0.
-template <bool W>
-__device__ Cand cand_block_reduce(Cand c)
-{
-    __shared__ float s_w[16];
-    __shared__ int32_t s_j[16], s_u[16], s_a[16];
-    c = cand_shfl_reduce<W>(c, 64);
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nw = (blockDim.x + 63) >> 6;
-    if (lane == 0) {
-        s_w[wave] = c.w;
-        s_j[wave] = c.j;
-        s_u[wave] = c.u;
-        s_a[wave] = c.alive;
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        c = lane < nw ? Cand{s_w[lane], s_j[lane], s_u[lane], s_a[lane]} : cand_none();
-        c = cand_shfl_reduce<W>(c, 64);
-    }
-    __syncthreads();
-    return c;
 }
 
 template <bool A8>
 __device__ __forceinline__ void load_entry(const Dev &d, int32_t j, int32_t &u, float &w)
 {
-    if (A8) {
-        // Rows are streamed once per round: non-temporal, so they do not evict
-        // the gathered state arrays from L2.
-        typedef int v2i __attribute__((ext_vector_type(2)));
-        const v2i e = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(d.adj + j));
-        u = e.x;
-        w = __int_as_float(e.y);
+    bool valid;
+    load_id<A8>(d, j, u, valid);
+    w = A8 ? (valid ? __builtin_nontemporal_load(d.sw + j) : -2.0f) : 1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// First-candidate scans.  Rows are stored in PRIORITY ORDER: for weighted
+// matching sorted by (weight descending, CSR position ascending) with the
+// weights that can never win (NaN, <= -1) last; for random matching in CSR
+// order.  The reference's choice -- the heaviest red neighbour, first in CSR
+// order on ties (gwRequest, src/matchgpu.cu:360-409), or the first red one
+// (grRequest, 286-327); likewise for proposers in gwRespond/grRespond -- is
+// then the FIRST candidate in row order, and a scan stops there.  In round 1
+// that is about two entries into a row instead of all of it.
+// ---------------------------------------------------------------------------
+
+// Candidate test of entry (u, w) of v's row; alive is set for the request
+// scan's dead test (an unmatched neighbour).
+template <bool W, bool RESPOND>
+__device__ __forceinline__ bool first_candidate(const Dev &d, int32_t v, int32_t u, bool valid, bool &alive)
+{
+    const uint8_t s = d.st[u];
+    bool cand;
+    if (RESPOND) {
+        cand = s == ST_BLUE && d.req[u] == v;
     } else {
-        u = __builtin_nontemporal_load(d.nbrs + j);
-        w = 1.0f;
+        alive = alive || s != ST_MATCHED;
+        cand = s == ST_RED;
     }
+    if (W) cand = cand && valid;
+    return cand;
 }
 
-// Request scan of one row by `stride` cooperating lanes starting at `lane`.
-template <bool W, bool A8>
-__device__ __forceinline__ Cand scan_request(const Dev &d, int2 rg, int lane, int stride)
-{
-    Cand c = cand_none();
-    for (int32_t j = rg.x + lane; j < rg.y; j += stride) {
-        int32_t u;
-        float w;
-        load_entry<A8>(d, j, u, w);
-        const uint8_t s = d.st[u];
-        c.alive |= (s != ST_MATCHED);
-        if (s == ST_RED) {
-            if (W) {
-                if (w > c.w) { c.w = w; c.j = j; c.u = u; }
-            } else {
-                c.j = j;
-                c.u = u;
-                break;  // later positions of this lane cannot win
-            }
-        }
-    }
-    return c;
-}
-
-// Respond scan (PULL): proposers are blue neighbours whose request names v.
-template <bool W, bool A8>
-__device__ __forceinline__ Cand scan_respond(const Dev &d, int32_t v, int2 rg, int lane, int stride)
-{
-    Cand c = cand_none();
-    for (int32_t j = rg.x + lane; j < rg.y; j += stride) {
-        int32_t u;
-        float w;
-        load_entry<A8>(d, j, u, w);
-        if (d.st[u] != ST_BLUE || d.req[u] != v) continue;
-        if (W) {
-            if (w > c.w) { c.w = w; c.j = j; c.u = u; }
-        } else {
-            c.j = j;
-            c.u = u;
-            break;
-        }
-    }
-    return c;
-}
-
-// 64-bit proposal key: order-preserving image of the weight (−0 folded onto +0,
-// so equal weights tie) above the complement of the proposer id (smaller id
+// 64-bit proposal key: order-preserving image of the weight (-0 folded onto +0,
+// so equal weights tie) above the complement of the proposer's id (smaller id
 // wins a tie, i.e. the first proposer of an ascending row).
 template <bool W>
 __device__ __forceinline__ unsigned long long proposal_key(float w, int32_t v)
@@ -262,16 +217,114 @@ __device__ __forceinline__ unsigned long long proposal_key(float w, int32_t v)
     return ((unsigned long long)hi << 32) | (unsigned long long)(0xFFFFFFFFu - (uint32_t)v);
 }
 
-template <bool W, bool PUSH>
-__device__ __forceinline__ void write_request(const Dev &d, int32_t v, const Cand &c, unsigned long long *resp)
+// Writes v's request (or, RESPOND, its accepted proposer) for winner entry
+// (u, w) or none (u < 0).
+template <bool W, bool PUSH, bool RESPOND>
+__device__ __forceinline__ void first_write(const Dev &d, int32_t v, int32_t vloc, int32_t u, float w, bool alive,
+                                            unsigned long long *resp)
 {
-    if (c.j != INT_MAX) {
-        d.req[v] = c.u;
-        if (PUSH) atomicMax(resp + c.u, proposal_key<W>(c.w, d.loc[v]));
+    if (u >= 0) {
+        d.req[v] = u;
+        if (!RESPOND && PUSH) atomicMax(resp + u, proposal_key<W>(w, vloc));
     } else {
-        d.req[v] = c.alive ? REQ_NONE : REQ_DEAD;
+        d.req[v] = (RESPOND || alive) ? REQ_NONE : REQ_DEAD;
     }
 }
+
+// G lanes per row, G entries per step; a group stops at its first candidate
+// (lowest lane of the first step that has one).  Wave-uniform control flow.
+template <bool W, bool A8, bool PUSH, bool RESPOND, int G>
+__device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp,
+                                  unsigned long long &nnz)
+{
+    constexpr int GPW = 64 / G;
+    const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
+    const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (gw * G));
+    const int step = (int)(blockDim.x >> 6) * GPW;
+    // The next step's entry is loaded one step ahead.
+    int first = (threadIdx.x >> 6) * GPW;
+    int4 nxt = first + gw < count ? load_ent(list + first + gw) : make_int4(0, 0, 0, 0);
+    for (; first < count; first += step) {
+        const int idx = first + gw;
+        const bool valid = idx < count;
+        const int4 ent = nxt;
+        if (first + step + gw < count) nxt = load_ent(list + first + step + gw);
+        const int32_t v = ent.x;
+        const int2 rg = valid ? make_int2(ent.y, ent.z) : make_int2(0, 0);
+        int32_t win_u = -1, win_j = -1;
+        bool alive = false, done = rg.x >= rg.y;
+        int32_t base = rg.x;
+        while (__ballot(!done) != 0ull) {
+            const int32_t j = base + gl;
+            int32_t u = 0;
+            bool cand = false;
+            if (!done && j < rg.y) {
+                bool ok;
+                load_id<A8>(d, j, u, ok);
+                cand = first_candidate<W, RESPOND>(d, v, u, ok, alive);
+            }
+            const unsigned long long m = __ballot(cand) & gmask;
+            const int src = m ? __ffsll((long long)m) - 1 : lane;
+            const int32_t su = __shfl(u, src);
+            if (!done && m) {
+                win_u = su;
+                win_j = base + (src & (G - 1));
+                done = true;
+            }
+            base += G;
+            if (base >= rg.y) done = true;
+        }
+        alive = (__ballot(alive) & gmask) != 0ull;
+        if (valid && gl == 0) {
+            // Only a posted proposal needs the winner's weight.
+            const float win_w = (W && A8 && PUSH && !RESPOND && win_u >= 0) ? d.sw[win_j] : 1.0f;
+            first_write<W, PUSH, RESPOND>(d, v, ent.w, win_u, win_w, alive, resp);
+            nnz += (unsigned long long)(rg.y - rg.x);
+        }
+    }
+}
+
+// Rows longer than the hub degree: one block per row, blockDim entries per
+// step, stopping at the step with the first candidate.
+template <bool W, bool A8, bool PUSH, bool RESPOND>
+__device__ void first_scan_hubs(const Dev &d, const int4 *list, int32_t count, int hb, int hub_blocks,
+                                unsigned long long *resp, unsigned long long &nnz)
+{
+    __shared__ int32_t s_first, s_alive;
+    for (int32_t idx = hb; idx < count; idx += hub_blocks) {
+        const int4 ent = list[idx];
+        const int32_t v = ent.x;
+        const int2 rg = make_int2(ent.y, ent.z);
+        if (threadIdx.x == 0) {
+            s_first = INT_MAX;
+            s_alive = 0;
+        }
+        __syncthreads();
+        for (int32_t base = rg.x; base < rg.y; base += blockDim.x) {
+            const int32_t j = base + threadIdx.x;
+            if (j < rg.y) {
+                int32_t u;
+                bool ok, alive = false;
+                load_id<A8>(d, j, u, ok);
+                if (first_candidate<W, RESPOND>(d, v, u, ok, alive)) atomicMin(&s_first, j);
+                if (alive) s_alive = 1;
+            }
+            __syncthreads();
+            const int32_t f = s_first;
+            __syncthreads();
+            if (f != INT_MAX) break;
+        }
+        if (threadIdx.x == 0) {
+            int32_t u = -1;
+            float w = 0.0f;
+            if (s_first != INT_MAX) load_entry<A8>(d, s_first, u, w);
+            first_write<W, PUSH, RESPOND>(d, v, ent.w, u, w, s_alive != 0, resp);
+            nnz += (unsigned long long)(rg.y - rg.x);
+        }
+        __syncthreads();
+    }
+}
+
 
 __global__ void k_clear(int32_t *ctr, unsigned long long *hub64, int n32, int n64)
 {
@@ -288,7 +341,7 @@ __device__ __forceinline__ unsigned long long *ctrb64_ptr(const Dev &d, int r, i
 // Appends value to this block's bucket for every lane with pred set; the
 // bucket counter lives in LDS, so no global atomic is involved.  Must be
 // reached by all lanes of the wavefront.
-__device__ __forceinline__ void bucket_append(bool pred, int32_t value, int32_t *bucket, int32_t *s_count)
+__device__ __forceinline__ void bucket_append(bool pred, int4 value, int4 *bucket, int32_t *s_count)
 {
     const unsigned long long mask = __ballot(pred);
     if (mask == 0ull) return;
@@ -329,34 +382,37 @@ template <bool PUSH>
 __global__ void k_init_select(Dev d)
 {
     __shared__ int32_t s_cnt[2];
-    const int parity = 1, b = blockIdx.x;
+    const int parity = 1, b = bucket_of_block(d);
     if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t seed = d.seeds[0];
     const int64_t lo = (int64_t)b * d.bcap, hi = min((int64_t)d.n, lo + d.bcap);
-    int32_t *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
-    int32_t *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
+    int4 *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
+    int4 *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
     unsigned long long deg_sum = 0, hub_deg_sum = 0;
     for (int64_t base = lo + (threadIdx.x & ~63); base < hi; base += blockDim.x) {
         const int32_t v = (int32_t)(base + (threadIdx.x & 63));
         const bool valid = v < hi;
         int cls = -1;
+        int4 ent = make_int4(v, 0, 0, 0);
         if (valid) {
-            const int c = colour_of((uint32_t)d.loc[v], seed, d.barrier);
+            const int32_t vloc = d.loc[v];
+            const int c = colour_of((uint32_t)vloc, seed, d.barrier);
             d.match[v] = c;
             d.st[v] = (uint8_t)c;
             if (d.vround) d.vround[v] = 255;
             const int2 rg = d.ranges[v];
+            ent = make_int4(v, rg.x, rg.y, vloc);
             const int hub = (rg.y - rg.x) > d.hub_deg ? 1 : 0;
             cls = 2 * c + hub;
             if (hub) hub_deg_sum += (unsigned long long)(rg.y - rg.x);
             else deg_sum += (unsigned long long)(rg.y - rg.x);
             if (PUSH && c == ST_RED) d.resp[(size_t)parity * d.n + v] = 0ull;
         }
-        bucket_append(cls == L_BLUE_LO, v, blue, s_cnt + 0);
-        bucket_append(cls == L_RED_LO, v, red, s_cnt + 1);
-        wave_append(cls == L_BLUE_HI, v, list_ptr(d, parity, L_BLUE_HI), d.ctr + 1 * CTR_STRIDE + L_BLUE_HI);
-        wave_append(cls == L_RED_HI, v, list_ptr(d, parity, L_RED_HI), d.ctr + 1 * CTR_STRIDE + L_RED_HI);
+        bucket_append(cls == L_BLUE_LO, ent, blue, s_cnt + 0);
+        bucket_append(cls == L_RED_LO, ent, red, s_cnt + 1);
+        wave_append(cls == L_BLUE_HI, ent, list_ptr(d, parity, L_BLUE_HI), d.ctr + 1 * CTR_STRIDE + L_BLUE_HI);
+        wave_append(cls == L_RED_HI, ent, list_ptr(d, parity, L_RED_HI), d.ctr + 1 * CTR_STRIDE + L_RED_HI);
     }
     if (d.count_nnz) {
         deg_sum = block_sum(deg_sum);
@@ -381,43 +437,16 @@ __global__ void k_request(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
     unsigned long long *resp = PUSH ? d.resp + (size_t)parity * d.n : nullptr;
+    unsigned long long nnz = 0;
     if ((int)blockIdx.x >= d.nb) {
-        const int hb = blockIdx.x - d.nb;
-        const int32_t count = d.ctr[r * CTR_STRIDE + L_BLUE_HI];
-        const int32_t *list = list_ptr(d, parity, L_BLUE_HI);
-        unsigned long long nnz = 0;
-        for (int32_t idx = hb; idx < count; idx += hub_blocks) {
-            const int32_t v = list[idx];
-            const int2 rg = d.ranges[v];
-            Cand c = scan_request<W, A8>(d, rg, threadIdx.x, blockDim.x);
-            c = cand_block_reduce<W>(c);
-            if (threadIdx.x == 0) {
-                write_request<W, PUSH>(d, v, c, resp);
-                nnz += (unsigned long long)(rg.y - rg.x);
-            }
-        }
+        first_scan_hubs<W, A8, PUSH, false>(d, list_ptr(d, parity, L_BLUE_HI), d.ctr[r * CTR_STRIDE + L_BLUE_HI],
+                                            blockIdx.x - d.nb, hub_blocks, resp, nnz);
         if (d.count_nnz && threadIdx.x == 0 && nnz) atomicAdd(d.hub64 + (size_t)r * 2 + 1, nnz);
         return;
     }
-    const int b = blockIdx.x;
-    const int32_t count = ctrb_ptr(d, r, 0)[b];
-    const int32_t *list = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
-    constexpr int GPW = 64 / G;  // groups per wavefront
-    const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
-    const int stride = (blockDim.x >> 6) * GPW;
-    unsigned long long nnz = 0;
-    for (int first = (threadIdx.x >> 6) * GPW; first < count; first += stride) {
-        const int idx = first + gw;
-        const bool valid = idx < count;
-        const int32_t v = valid ? list[idx] : 0;
-        const int2 rg = valid ? d.ranges[v] : make_int2(0, 0);
-        Cand c = scan_request<W, A8>(d, rg, gl, G);
-        if (G > 1) c = cand_shfl_reduce<W>(c, G);
-        if (valid && gl == 0) {
-            write_request<W, PUSH>(d, v, c, resp);
-            nnz += (unsigned long long)(rg.y - rg.x);
-        }
-    }
+    const int b = bucket_of_block(d);
+    first_scan_bucket<W, A8, PUSH, false, G>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap,
+                                             ctrb_ptr(d, r, 0)[b], resp, nnz);
     if (d.count_nnz) {
         nnz = block_sum(nnz);
         if (threadIdx.x == 0) ctrb64_ptr(d, r, 1)[b] = nnz;
@@ -435,7 +464,6 @@ __global__ void k_request(Dev d, int r, int hub_blocks)
 // scan from -1.0.  Only two dependent memory steps remain per entry (row
 // entry, neighbour state), whatever the degree distribution.
 // ---------------------------------------------------------------------------
-constexpr int TILE_U = 4;  // row entries each thread has in flight
 
 // Exclusive block scan of x; total in *s_total.  Reached by all threads.
 __device__ int32_t block_exclusive_scan(int32_t x, int32_t *s_wave, int32_t *s_total)
@@ -481,8 +509,9 @@ __device__ __forceinline__ uint32_t order_bits(float w)
 }
 
 // RESPOND == false: request scan of blue rows; true: respond scan of red rows.
-template <bool W, bool A8, bool PUSH, bool RESPOND>
-__device__ void tile_scan_bucket(const Dev &d, const int32_t *list, int32_t count, unsigned long long *resp)
+// U: row entries each thread has in flight.
+template <bool W, bool A8, bool PUSH, bool RESPOND, int U>
+__device__ void tile_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp)
 {
     extern __shared__ unsigned long long s_dyn[];
     const int T = blockDim.x;
@@ -498,8 +527,9 @@ __device__ void tile_scan_bucket(const Dev &d, const int32_t *list, int32_t coun
         const int tn = min(T, count - t0);
         int32_t deg = 0;
         if (tid < tn) {
-            const int32_t v = __builtin_nontemporal_load(list + t0 + tid);
-            const int2 rg = d.ranges[v];
+            const int4 ent = load_ent(list + t0 + tid);
+            const int32_t v = ent.x;
+            const int2 rg = make_int2(ent.y, ent.z);
             s_v[tid] = v;
             s_beg[tid] = rg.x;
             deg = rg.y - rg.x;
@@ -509,11 +539,11 @@ __device__ void tile_scan_bucket(const Dev &d, const int32_t *list, int32_t coun
         s_off[tid] = block_exclusive_scan(deg, s_wave, s_total);
         __syncthreads();
         const int32_t E = *s_total;
-        for (int32_t e0 = 0; e0 < E; e0 += T * TILE_U) {
-            int32_t u[TILE_U], k[TILE_U], pos[TILE_U];
-            float w[TILE_U];
+        for (int32_t e0 = 0; e0 < E; e0 += T * U) {
+            int32_t u[U], k[U], pos[U];
+            float w[U];
 #pragma unroll
-            for (int q = 0; q < TILE_U; ++q) {
+            for (int q = 0; q < U; ++q) {
                 const int32_t e = e0 + q * T + tid;
                 k[q] = -1;
                 if (e < E) {
@@ -522,15 +552,15 @@ __device__ void tile_scan_bucket(const Dev &d, const int32_t *list, int32_t coun
                     load_entry<A8>(d, s_beg[k[q]] + pos[q], u[q], w[q]);
                 }
             }
-            uint8_t s[TILE_U];
-            int32_t rq[TILE_U];
+            uint8_t s[U];
+            int32_t rq[U];
 #pragma unroll
-            for (int q = 0; q < TILE_U; ++q) {
+            for (int q = 0; q < U; ++q) {
                 s[q] = k[q] >= 0 ? d.st[u[q]] : ST_MATCHED;
                 if (RESPOND) rq[q] = (k[q] >= 0 && s[q] == ST_BLUE) ? d.req[u[q]] : -1;
             }
 #pragma unroll
-            for (int q = 0; q < TILE_U; ++q) {
+            for (int q = 0; q < U; ++q) {
                 if (k[q] < 0) continue;
                 bool cand;
                 if (RESPOND) {
@@ -568,63 +598,45 @@ __device__ void tile_scan_bucket(const Dev &d, const int32_t *list, int32_t coun
     }
 }
 
-template <bool W, bool A8, bool PUSH>
+template <bool W, bool A8, bool PUSH, int U>
 __global__ void k_request_tile(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
     unsigned long long *resp = PUSH ? d.resp + (size_t)parity * d.n : nullptr;
     if ((int)blockIdx.x >= d.nb) {
-        const int hb = blockIdx.x - d.nb;
-        const int32_t count = d.ctr[r * CTR_STRIDE + L_BLUE_HI];
-        const int32_t *list = list_ptr(d, parity, L_BLUE_HI);
         unsigned long long nnz = 0;
-        for (int32_t idx = hb; idx < count; idx += hub_blocks) {
-            const int32_t v = list[idx];
-            const int2 rg = d.ranges[v];
-            Cand c = scan_request<W, A8>(d, rg, threadIdx.x, blockDim.x);
-            c = cand_block_reduce<W>(c);
-            if (threadIdx.x == 0) {
-                write_request<W, PUSH>(d, v, c, resp);
-                nnz += (unsigned long long)(rg.y - rg.x);
-            }
-        }
+        first_scan_hubs<W, A8, PUSH, false>(d, list_ptr(d, parity, L_BLUE_HI), d.ctr[r * CTR_STRIDE + L_BLUE_HI],
+                                            blockIdx.x - d.nb, hub_blocks, resp, nnz);
         if (d.count_nnz && threadIdx.x == 0 && nnz) atomicAdd(d.hub64 + (size_t)r * 2 + 1, nnz);
         return;
     }
-    const int b = blockIdx.x;
+    const int b = bucket_of_block(d);
     const int32_t count = ctrb_ptr(d, r, 0)[b];
-    const int32_t *list = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
-    tile_scan_bucket<W, A8, PUSH, false>(d, list, count, resp);
+    const int4 *list = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
+    tile_scan_bucket<W, A8, PUSH, false, U>(d, list, count, resp);
     if (d.count_nnz) {
         unsigned long long nnz = 0;
         for (int i = threadIdx.x; i < count; i += blockDim.x) {
-            const int2 rg = d.ranges[list[i]];
-            nnz += (unsigned long long)(rg.y - rg.x);
+            const int4 e = list[i];
+            nnz += (unsigned long long)(e.z - e.y);
         }
         nnz = block_sum(nnz);
         if (threadIdx.x == 0) ctrb64_ptr(d, r, 1)[b] = nnz;
     }
 }
 
-template <bool W, bool A8>
+template <bool W, bool A8, int U>
 __global__ void k_respond_tile(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
     if ((int)blockIdx.x >= d.nb) {
-        const int hb = blockIdx.x - d.nb;
-        const int32_t count = d.ctr[r * CTR_STRIDE + L_RED_HI];
-        const int32_t *list = list_ptr(d, parity, L_RED_HI);
-        for (int32_t idx = hb; idx < count; idx += hub_blocks) {
-            const int32_t v = list[idx];
-            const int2 rg = d.ranges[v];
-            Cand c = scan_respond<W, A8>(d, v, rg, threadIdx.x, blockDim.x);
-            c = cand_block_reduce<W>(c);
-            if (threadIdx.x == 0) d.req[v] = c.j != INT_MAX ? c.u : REQ_NONE;
-        }
+        unsigned long long nnz = 0;
+        first_scan_hubs<W, A8, false, true>(d, list_ptr(d, parity, L_RED_HI), d.ctr[r * CTR_STRIDE + L_RED_HI],
+                                            blockIdx.x - d.nb, hub_blocks, nullptr, nnz);
         return;
     }
-    const int b = blockIdx.x;
-    tile_scan_bucket<W, A8, false, true>(d, list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap,
+    const int b = bucket_of_block(d);
+    tile_scan_bucket<W, A8, false, true, U>(d, list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap,
                                          ctrb_ptr(d, r, 1)[b], nullptr);
 }
 
@@ -635,34 +647,15 @@ template <bool W, bool A8, int G>
 __global__ void k_respond(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
+    unsigned long long nnz = 0;
     if ((int)blockIdx.x >= d.nb) {
-        const int hb = blockIdx.x - d.nb;
-        const int32_t count = d.ctr[r * CTR_STRIDE + L_RED_HI];
-        const int32_t *list = list_ptr(d, parity, L_RED_HI);
-        for (int32_t idx = hb; idx < count; idx += hub_blocks) {
-            const int32_t v = list[idx];
-            const int2 rg = d.ranges[v];
-            Cand c = scan_respond<W, A8>(d, v, rg, threadIdx.x, blockDim.x);
-            c = cand_block_reduce<W>(c);
-            if (threadIdx.x == 0) d.req[v] = c.j != INT_MAX ? c.u : REQ_NONE;
-        }
+        first_scan_hubs<W, A8, false, true>(d, list_ptr(d, parity, L_RED_HI), d.ctr[r * CTR_STRIDE + L_RED_HI],
+                                            blockIdx.x - d.nb, hub_blocks, nullptr, nnz);
         return;
     }
-    const int b = blockIdx.x;
-    const int32_t count = ctrb_ptr(d, r, 1)[b];
-    const int32_t *list = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
-    constexpr int GPW = 64 / G;
-    const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
-    const int stride = (blockDim.x >> 6) * GPW;
-    for (int first = (threadIdx.x >> 6) * GPW; first < count; first += stride) {
-        const int idx = first + gw;
-        const bool valid = idx < count;
-        const int32_t v = valid ? list[idx] : 0;
-        const int2 rg = valid ? d.ranges[v] : make_int2(0, 0);
-        Cand c = scan_respond<W, A8>(d, v, rg, gl, G);
-        if (G > 1) c = cand_shfl_reduce<W>(c, G);
-        if (valid && gl == 0) d.req[v] = c.j != INT_MAX ? c.u : REQ_NONE;
-    }
+    const int b = bucket_of_block(d);
+    first_scan_bucket<W, A8, false, true, G>(d, list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap,
+                                             ctrb_ptr(d, r, 1)[b], nullptr, nnz);
 }
 
 // Writes the outcome of round r for vertex v (gMatch, src/matchgpu.cu:260-283)
@@ -750,15 +743,15 @@ template <bool PUSH>
 __global__ void k_match_select(Dev d, int r, int do_select)
 {
     __shared__ int32_t s_cnt[2];
-    const int parity = r & 1, nparity = parity ^ 1, b = blockIdx.x;
+    const int parity = r & 1, nparity = parity ^ 1, b = bucket_of_block(d);
     if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
     __syncthreads();
     const int32_t c0 = ctrb_ptr(d, r, 0)[b], c1 = ctrb_ptr(d, r, 1)[b];
     const int32_t total = c0 + c1;
-    const int32_t *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
-    const int32_t *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
-    int32_t *nblue = list_ptr(d, nparity, L_BLUE_LO) + (size_t)b * d.bcap;
-    int32_t *nred = list_ptr(d, nparity, L_RED_LO) + (size_t)b * d.bcap;
+    const int4 *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
+    const int4 *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
+    int4 *nblue = list_ptr(d, nparity, L_BLUE_LO) + (size_t)b * d.bcap;
+    int4 *nred = list_ptr(d, nparity, L_RED_LO) + (size_t)b * d.bcap;
     const unsigned long long *resp = PUSH ? d.resp + (size_t)parity * d.n : nullptr;
     unsigned long long *nresp = PUSH ? d.resp + (size_t)nparity * d.n : nullptr;
     const uint32_t seed = do_select ? d.seeds[r] : 0u;
@@ -767,6 +760,7 @@ __global__ void k_match_select(Dev d, int r, int do_select)
     // MU vertices per thread: all loads of a step are issued before any store.
     constexpr int MU = 4;
     for (int base = threadIdx.x & ~63; base < total; base += blockDim.x * MU) {
+        int4 e[MU];
         int32_t v[MU], a[MU], vl[MU], pl[MU];
         unsigned long long k[MU];
         int col[MU];
@@ -774,21 +768,19 @@ __global__ void k_match_select(Dev d, int r, int do_select)
         for (int q = 0; q < MU; ++q) {
             const int t = base + q * blockDim.x + (threadIdx.x & 63);
             col[q] = -1;
-            v[q] = 0;
+            e[q] = make_int4(0, 0, 0, 0);
             if (t < total) {
                 col[q] = t < c0 ? ST_BLUE : ST_RED;
-                v[q] = __builtin_nontemporal_load(t < c0 ? blue + t : red + (t - c0));
+                e[q] = load_ent(t < c0 ? blue + t : red + (t - c0));
             }
+            v[q] = e[q].x;
+            vl[q] = e[q].w;
         }
 #pragma unroll
         for (int q = 0; q < MU; ++q) {
             a[q] = -1;
             k[q] = 0ull;
-            vl[q] = 0;
-            if (col[q] >= 0) {
-                partner_words<PUSH>(d, v[q], col[q], resp, a[q], k[q]);
-                vl[q] = d.loc[v[q]];
-            }
+            if (col[q] >= 0) partner_words<PUSH>(d, v[q], col[q], resp, a[q], k[q]);
         }
 #pragma unroll
         for (int q = 0; q < MU; ++q) {
@@ -804,43 +796,37 @@ __global__ void k_match_select(Dev d, int r, int do_select)
                 n_matched += partner >= 0;
                 n_dead += dd;
                 cls = settle_vertex<PUSH>(d, v[q], vl[q], col[q], 0, partner, dd, r, do_select, seed, nresp);
-                if (cls >= 0 && d.count_nnz) {
-                    const int2 rg = d.ranges[v[q]];
-                    deg_sum += (unsigned long long)(rg.y - rg.x);
-                }
+                if (cls >= 0 && d.count_nnz) deg_sum += (unsigned long long)(e[q].z - e[q].y);
             }
-            bucket_append(cls == L_BLUE_LO, v[q], nblue, s_cnt + 0);
-            bucket_append(cls == L_RED_LO, v[q], nred, s_cnt + 1);
+            bucket_append(cls == L_BLUE_LO, e[q], nblue, s_cnt + 0);
+            bucket_append(cls == L_RED_LO, e[q], nred, s_cnt + 1);
         }
     }
     // hub rows: a share of the global hub lists
     const int32_t h0 = d.ctr[r * CTR_STRIDE + L_BLUE_HI], h1 = d.ctr[r * CTR_STRIDE + L_RED_HI];
-    const int32_t *hblue = list_ptr(d, parity, L_BLUE_HI), *hred = list_ptr(d, parity, L_RED_HI);
+    const int4 *hblue = list_ptr(d, parity, L_BLUE_HI), *hred = list_ptr(d, parity, L_RED_HI);
     for (int64_t base = (int64_t)b * blockDim.x + (threadIdx.x & ~63); base < (int64_t)h0 + h1;
          base += (int64_t)d.nb * blockDim.x) {
         const int64_t t = base + (threadIdx.x & 63);
         int cls = -1;
-        int32_t v = 0;
+        int4 ent = make_int4(0, 0, 0, 0);
         if (t < (int64_t)h0 + h1) {
             const int colour = t < h0 ? ST_BLUE : ST_RED;
-            v = t < h0 ? hblue[t] : hred[t - h0];
+            ent = t < h0 ? hblue[t] : hred[t - h0];
+            const int32_t v = ent.x, vloc = ent.w;
             int32_t a, pl;
             unsigned long long k;
             partner_words<PUSH>(d, v, colour, resp, a, k);
             partner_words2<PUSH>(d, colour, resp, a, k, pl);
-            const int32_t vloc = d.loc[v];
             bool dd = false;
             const int32_t partner = partner_of<PUSH>(v, vloc, colour, a, k, pl, dd);
             n_matched += partner >= 0;
             n_dead += dd;
             cls = settle_vertex<PUSH>(d, v, vloc, colour, 1, partner, dd, r, do_select, seed, nresp);
-            if (cls >= 0 && d.count_nnz) {
-                const int2 rg = d.ranges[v];
-                hub_deg_sum += (unsigned long long)(rg.y - rg.x);
-            }
+            if (cls >= 0 && d.count_nnz) hub_deg_sum += (unsigned long long)(ent.z - ent.y);
         }
-        wave_append(cls == L_BLUE_HI, v, list_ptr(d, nparity, L_BLUE_HI), d.ctr + (r + 1) * CTR_STRIDE + L_BLUE_HI);
-        wave_append(cls == L_RED_HI, v, list_ptr(d, nparity, L_RED_HI), d.ctr + (r + 1) * CTR_STRIDE + L_RED_HI);
+        wave_append(cls == L_BLUE_HI, ent, list_ptr(d, nparity, L_BLUE_HI), d.ctr + (r + 1) * CTR_STRIDE + L_BLUE_HI);
+        wave_append(cls == L_RED_HI, ent, list_ptr(d, nparity, L_RED_HI), d.ctr + (r + 1) * CTR_STRIDE + L_RED_HI);
     }
     n_matched = block_sum(n_matched);
     n_dead = block_sum(n_dead);
@@ -897,11 +883,12 @@ __global__ void k_reduce_counts(Dev d, int r0, int with_nnz)
     }
 }
 
-// Writes the state array back in the caller's vertex numbering.
-__global__ void k_unpermute(const int32_t *match, const int32_t *old_of, int32_t *out, int32_t n)
+// Writes the state array back in the caller's vertex numbering: coalesced
+// writes, reads gathered from the (L2/MALL-resident) state array.
+__global__ void k_unpermute(const int32_t *match, const int32_t *new_of, int32_t *out, int32_t n)
 {
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (int64_t)gridDim.x * blockDim.x)
-        out[old_of[v]] = match[v];
+    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (int64_t)gridDim.x * blockDim.x)
+        out[o] = match[new_of[o]];
 }
 
 // Rounds each batched graph would execute alone: one plus the last round in
@@ -935,7 +922,7 @@ __global__ void k_graph_rounds(const uint8_t *vround, const int32_t *voff, int32
 
 // Key of a captured launch sequence: every value baked into its kernel arguments.
 struct ExecKey {
-    int32_t R_run, W, A8, PUSH, G, block, hub_deg, nb, bcap, maximal, hub_blocks, count_nnz, r0, r1;
+    int32_t R_run, W, A8, PUSH, G, U, block, hub_deg, nb, bcap, maximal, hub_blocks, count_nnz, r0, r1;
     uint32_t barrier;
 };
 struct CachedExec {
@@ -957,12 +944,13 @@ struct mm_graph {
     int64_t deg_hist[33] = {0};  // deg_hist[b]: vertices with degree in [2^b, 2^(b+1)) (b=0: deg 1)
     int2 *d_ranges = nullptr;
     int32_t *d_nbrs = nullptr;
-    int2 *d_adj = nullptr;
+    int32_t *d_sid = nullptr;
+    float *d_sw = nullptr;
     int32_t *d_match = nullptr;
     int32_t *d_req = nullptr;
     uint8_t *d_st = nullptr;
     unsigned long long *d_resp = nullptr;
-    int32_t *d_lists = nullptr;
+    int4 *d_lists = nullptr;
     int32_t *d_ctr = nullptr;
     int32_t *d_ctrb = nullptr;
     unsigned long long *d_ctr64 = nullptr;
@@ -976,7 +964,7 @@ struct mm_graph {
     // batched graphs
     int32_t n_graphs = 0;
     int32_t *d_loc = nullptr;        // graph-local original id per internal vertex
-    int32_t *d_old_of = nullptr;     // original (global) id per internal vertex, when relabelled
+    int32_t *d_new_of = nullptr;     // internal id per original (global) id, when relabelled
     int32_t *d_match_out = nullptr;  // result in the caller's numbering, when relabelled
     bool relabelled = false;
     int32_t *d_voff = nullptr;
@@ -1023,10 +1011,10 @@ void release(mm_graph *g)
     if (!g) return;
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
-    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_match, g->d_req,   g->d_st,
+    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_sid,    g->d_sw,    g->d_match, g->d_req,   g->d_st,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
-                    g->d_old_of, g->d_match_out};
+                    g->d_new_of, g->d_match_out};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (g->h_ctr) hipHostFree(g->h_ctr);
@@ -1156,11 +1144,10 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         return fail(mm::set_error(MM_ERR_HIP, "hipEventCreate failed"));
     const size_t nn = (size_t)n;
     if ((rc = dev_alloc(g, &g->d_ranges, nn, "neighbour ranges")) != MM_OK) return fail(rc);
-    if (g->weighted) {
-        if ((rc = dev_alloc(g, &g->d_adj, (size_t)nnz, "weighted rows")) != MM_OK) return fail(rc);
-    } else {
-        if ((rc = dev_alloc(g, &g->d_nbrs, (size_t)nnz, "neighbours")) != MM_OK) return fail(rc);
-    }
+    if ((rc = dev_alloc(g, &g->d_nbrs, (size_t)nnz, "neighbours")) != MM_OK) return fail(rc);
+    if (g->weighted && ((rc = dev_alloc(g, &g->d_sid, (size_t)nnz, "sorted rows")) != MM_OK ||
+                        (rc = dev_alloc(g, &g->d_sw, (size_t)nnz, "sorted weights")) != MM_OK))
+        return fail(rc);
     if ((rc = dev_alloc(g, &g->d_match, nn, "match")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_req, nn, "requests")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_st, nn, "state")) != MM_OK) return fail(rc);
@@ -1169,6 +1156,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     g->nb_max = std::max(1, g->num_cu * 32);
     g->list_cap = (int64_t)n + g->nb_max;
     if ((rc = dev_alloc(g, &g->d_lists, 2 * NLISTS * (size_t)g->list_cap, "work lists")) != MM_OK) return fail(rc);
+    static_assert(sizeof(*g->d_lists) == 16, "work-list entries are 16 bytes");
     if ((rc = dev_alloc(g, &g->d_ctr, (size_t)ROUND_SLOTS * CTR_STRIDE, "counters")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_ctrb, (size_t)ROUND_SLOTS * 4 * g->nb_max, "bucket counters")) != MM_OK)
         return fail(rc);
@@ -1181,9 +1169,14 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         hipHostMalloc((void **)&g->h_ctr64, sizeof(unsigned long long) * ROUND_SLOTS * CTR64_STRIDE) != hipSuccess)
         return fail(mm::set_error(MM_ERR_NOMEM, "pinned host allocation failed"));
     if ((rc = dev_alloc(g, &g->d_loc, nn, "original ids")) != MM_OK) return fail(rc);
-    // Device copy of the CSR, rows in the internal order: each row keeps its
-    // entries in their original order (CSR position decides ties), neighbour
-    // ids are renumbered, weights are interleaved with the ids.
+    // Device copies of the CSR, rows in the internal vertex order, neighbour
+    // ids renumbered:
+    //   nbrs: each row in CSR order (random matching: first red neighbour);
+    //   adj (weighted graphs): {id, weight} pairs sorted by weight descending,
+    //     CSR position ascending, with the weights that never win (NaN and
+    //     <= -1, the reference scans with '>' from -1.0) last in CSR order --
+    //     so the heaviest candidate, first in CSR order on ties, is the first
+    //     candidate in the row.
     g->relabelled = relabel != 0 && n > 1;
     std::vector<int32_t> order;
     if (g->relabelled) order = bfs_order(n, ranges, nbrs);
@@ -1203,11 +1196,15 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         h_loc[(size_t)i] = o - (vbase_old ? vbase_old[(size_t)o] : 0);
         off += deg;
     }
-    std::vector<int32_t> h_nbrs;
-    std::vector<int2> h_adj;
-    if (g->weighted) h_adj.resize((size_t)nnz);
-    else h_nbrs.resize((size_t)nnz);
+    std::vector<int32_t> h_nbrs((size_t)nnz);
+    std::vector<int32_t> h_sid;
+    std::vector<float> h_sw;
+    if (g->weighted) {
+        h_sid.resize((size_t)nnz);
+        h_sw.resize((size_t)nnz);
+    }
     mm::parallel_for(n, [&](int64_t b, int64_t e, unsigned) {
+        std::vector<int32_t> perm;
         for (int64_t i = b; i < e; ++i) {
             const int32_t o = old_of(i);
             const int32_t src = ranges[2 * (size_t)o];
@@ -1215,14 +1212,22 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
             const int32_t dst = h_ranges[2 * (size_t)i];
             for (int32_t k = 0; k < deg; ++k) {
                 const int32_t u = nbrs[src + k];
-                const int32_t un = g->relabelled ? inv[(size_t)u] : u;
-                if (g->weighted) {
-                    int32_t wb;
-                    std::memcpy(&wb, w + src + k, sizeof(wb));
-                    h_adj[(size_t)dst + k] = make_int2(un, wb);
-                } else {
-                    h_nbrs[(size_t)dst + k] = un;
-                }
+                h_nbrs[(size_t)dst + k] = g->relabelled ? inv[(size_t)u] : u;
+            }
+            if (!g->weighted) continue;
+            perm.resize((size_t)deg);
+            for (int32_t k = 0; k < deg; ++k) perm[(size_t)k] = k;
+            const float *wr = w + src;
+            std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) {
+                const bool vx = wr[x] > -1.0f, vy = wr[y] > -1.0f;
+                if (vx != vy) return vx;
+                return vx && wr[x] > wr[y];
+            });
+            for (int32_t k = 0; k < deg; ++k) {
+                const float wk = wr[perm[(size_t)k]];
+                const int32_t id = h_nbrs[(size_t)dst + perm[(size_t)k]];
+                h_sid[(size_t)dst + k] = wk > -1.0f ? id : ~id;
+                h_sw[(size_t)dst + k] = wk;
             }
         }
     });
@@ -1230,24 +1235,30 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
                   hipMemcpy(g->d_loc, h_loc.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess))
         return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
     if (nnz > 0) {
-        const hipError_t e = g->weighted
-                                 ? hipMemcpy(g->d_adj, h_adj.data(), sizeof(int2) * (size_t)nnz, hipMemcpyHostToDevice)
-                                 : hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz,
-                                             hipMemcpyHostToDevice);
-        if (e != hipSuccess) return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+        if (hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
+            (g->weighted &&
+             (hipMemcpy(g->d_sid, h_sid.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(g->d_sw, h_sw.data(), sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess)))
+            return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
     }
     if (g->relabelled) {
-        if ((rc = dev_alloc(g, &g->d_old_of, nn, "vertex order")) != MM_OK) return fail(rc);
+        if ((rc = dev_alloc(g, &g->d_new_of, nn, "vertex order")) != MM_OK) return fail(rc);
         if ((rc = dev_alloc(g, &g->d_match_out, nn, "match")) != MM_OK) return fail(rc);
-        if (hipMemcpy(g->d_old_of, order.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(g->d_new_of, inv.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess)
             return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
     }
     *out = g;
     return MM_OK;
 }
 
-// 0 selects the entry-parallel tile kernels; a power of two selects G lanes per row.
-int pick_lanes(const mm_graph *, int requested) { return requested > 0 ? requested : 0; }
+// Lanes per row of the first-candidate scans (0: auto = 4); MM_LANES_TILES
+// selects the entry-parallel tile kernels (G = 0 internally).
+int pick_lanes(const mm_graph *, int requested)
+{
+    if (requested == MM_LANES_TILES) return 0;
+    if (requested < 0) return -1;
+    return requested > 0 ? requested : 4;
+}
 
 int64_t hubs_above(const mm_graph *g, int32_t thr)
 {
@@ -1260,7 +1271,7 @@ int64_t hubs_above(const mm_graph *g, int32_t thr)
 
 struct RunCfg {
     bool W, A8, PUSH;
-    int G, block, hub_deg, nb, bcap, hub_blocks;
+    int G, U, block, hub_deg, nb, bcap, hub_blocks;
 };
 
 template <bool W, bool A8, bool PUSH, int G>
@@ -1308,8 +1319,11 @@ void launch_respond_g(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 template <bool W, bool A8, bool PUSH>
 void launch_request_tile(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_request_tile<W, A8, PUSH>), dim3(c.nb + c.hub_blocks), dim3(c.block),
-                       tile_smem_bytes(c.block), s, d, r, c.hub_blocks);
+    const dim3 grid(c.nb + c.hub_blocks), blk(c.block);
+    const size_t sm = tile_smem_bytes(c.block);
+    if (c.U == 4) hipLaunchKernelGGL((k_request_tile<W, A8, PUSH, 4>), grid, blk, sm, s, d, r, c.hub_blocks);
+    else if (c.U == 8) hipLaunchKernelGGL((k_request_tile<W, A8, PUSH, 8>), grid, blk, sm, s, d, r, c.hub_blocks);
+    else hipLaunchKernelGGL((k_request_tile<W, A8, PUSH, 16>), grid, blk, sm, s, d, r, c.hub_blocks);
 }
 
 void launch_request(const RunCfg &c, const Dev &d, int r, hipStream_t s)
@@ -1336,8 +1350,11 @@ void launch_request(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 template <bool W, bool A8>
 void launch_respond_tile(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_respond_tile<W, A8>), dim3(c.nb + c.hub_blocks), dim3(c.block), tile_smem_bytes(c.block), s,
-                       d, r, c.hub_blocks);
+    const dim3 grid(c.nb + c.hub_blocks), blk(c.block);
+    const size_t sm = tile_smem_bytes(c.block);
+    if (c.U == 4) hipLaunchKernelGGL((k_respond_tile<W, A8, 4>), grid, blk, sm, s, d, r, c.hub_blocks);
+    else if (c.U == 8) hipLaunchKernelGGL((k_respond_tile<W, A8, 8>), grid, blk, sm, s, d, r, c.hub_blocks);
+    else hipLaunchKernelGGL((k_respond_tile<W, A8, 16>), grid, blk, sm, s, d, r, c.hub_blocks);
 }
 
 void launch_respond(const RunCfg &c, const Dev &d, int r, hipStream_t s)
@@ -1366,19 +1383,24 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     if (G < 0 || G > 64 || (G & (G - 1))) return mm::set_error(MM_ERR_INVALID, "mm_match: bad lanes_per_vertex");
     RunCfg c;
     c.W = o->kind == MM_WEIGHTED;
-    c.A8 = g->weighted != 0;
+    // Weighted matching reads the weight-sorted rows; random matching the CSR-order ids.
+    c.A8 = c.W && g->weighted != 0;
     const bool push_ok = c.W && g->weighted ? g->push_ok_weights : g->push_ok_struct;
     if (o->respond == MM_RESPOND_PUSH && !push_ok)
         return mm::set_error(MM_ERR_INVALID, "mm_match: PUSH needs a symmetric CSR with ascending rows and symmetric weights");
     if (o->respond < 0 || o->respond > MM_RESPOND_PUSH) return mm::set_error(MM_ERR_INVALID, "mm_match: bad respond mode");
     c.PUSH = o->respond == MM_RESPOND_PUSH || (o->respond == MM_RESPOND_AUTO && push_ok);
     c.G = G;
+    c.U = o->entries_per_thread > 0 ? o->entries_per_thread : 4;
+    if (c.U != 4 && c.U != 8 && c.U != 16)
+        return mm::set_error(MM_ERR_INVALID, "mm_match: entries_per_thread must be 4, 8 or 16");
     c.block = block;
     c.hub_deg = o->hub_degree > 0 ? o->hub_degree : 256;
     const int blocks_per_cu = std::max(1, 2048 / block);
     // One bucket per block of the per-bucket kernels, about 2 * block ids each.
     c.nb = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)g->n + 2 * block - 1) / (2 * block),
                                                         std::min<int64_t>(g->nb_max, (int64_t)g->num_cu * blocks_per_cu)));
+    c.nb = (c.nb + 7) & ~7;  // bucket_of_block deals buckets to XCDs in eighths
     c.bcap = (int)(((int64_t)g->n + c.nb - 1) / c.nb);
     c.hub_blocks = (int)std::min<int64_t>(hubs_above(g, c.hub_deg), (int64_t)g->num_cu * blocks_per_cu);
 
@@ -1403,7 +1425,8 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.bcap = c.bcap;
     d.ranges = g->d_ranges;
     d.nbrs = g->d_nbrs;
-    d.adj = g->d_adj;
+    d.sid = g->d_sid;
+    d.sw = g->d_sw;
     d.match = g->d_match;
     d.req = g->d_req;
     d.st = g->d_st;
@@ -1482,7 +1505,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     // Replays the captured launch sequence of rounds [r0, r1], capturing it on first use.
     ExecKey key;
     std::memset(&key, 0, sizeof(key));
-    key.R_run = R_run; key.W = c.W; key.A8 = c.A8; key.PUSH = c.PUSH; key.G = c.G; key.block = c.block;
+    key.R_run = R_run; key.W = c.W; key.A8 = c.A8; key.PUSH = c.PUSH; key.G = c.G; key.U = c.U; key.block = c.block;
     key.hub_deg = c.hub_deg; key.nb = c.nb; key.bcap = c.bcap; key.maximal = maximal;
     key.hub_blocks = c.hub_blocks; key.barrier = o->barrier; key.count_nnz = d.count_nnz;
     auto run_chunk = [&](int r0, int r1) -> int32_t {
@@ -1545,7 +1568,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     // The result in the caller's numbering is part of the matching.
     if (g->relabelled && g->n > 0)
         hipLaunchKernelGGL(k_unpermute, dim3(std::min<int64_t>(((int64_t)g->n + 255) / 256, (int64_t)g->num_cu * 8)),
-                           dim3(256), 0, g->stream, g->d_match, g->d_old_of, g->d_match_out, g->n);
+                           dim3(256), 0, g->stream, g->d_match, g->d_new_of, g->d_match_out, g->n);
     MM_HIP_TRY(hipEventRecord(g->ev_t2, g->stream), "hipEventRecord");
     // Totals of every executed round (a maximal run's last, empty round has none).
     const int r_stats = g->n > 0 ? std::min(rounds, R_run) : 0;

@@ -30,6 +30,7 @@ MM_NR_MAX_MATCH_ROUNDS = 255
 MM_MAX_ROUNDS = 255
 MM_DEFAULT_BARRIER = 0x88B81733
 MM_CREATE_KEEP_ORDER = 1
+MM_LANES_TILES = -1
 MM_KERNEL_CLASSES = 5
 KERNEL_CLASS_NAMES = ("init_select", "request", "respond", "match_select", "counters")
 
@@ -55,6 +56,7 @@ class _Opts(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("lanes_per_vertex", ctypes.c_int32),
         ("hub_degree", ctypes.c_int32),
+        ("entries_per_thread", ctypes.c_int32),
         ("respond", ctypes.c_int32),
         ("profile", ctypes.c_int32),
         ("graph_mode", ctypes.c_int32),
@@ -274,7 +276,7 @@ class MatchResult:
 
 
 def _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond, profile,
-               graph_mode=0):
+               graph_mode=0, unroll=0):
     seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
     o = _Opts()
     o.kind = kind
@@ -289,6 +291,7 @@ def _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub
     o.respond = respond
     o.profile = int(profile)
     o.graph_mode = int(graph_mode)
+    o.entries_per_thread = int(unroll)
     return o, seeds
 
 
@@ -329,13 +332,13 @@ class Graph:
     def match(self, kind: int = MM_WEIGHTED, maximal: bool = True, seeds=None, max_rounds: int = 0,
               barrier: int = MM_DEFAULT_BARRIER, block_size: int = 0, lanes: int = 0, hub_degree: int = 0,
               respond: int = MM_RESPOND_AUTO, profile: bool = False, fetch: bool = True,
-              stats: bool = True, graph_mode: int = 0) -> MatchResult | None:
+              stats: bool = True, graph_mode: int = 0, unroll: int = 0) -> MatchResult | None:
         """Runs one matching.  fetch=False leaves the match array on the device;
         stats=False skips the counter read-back and returns None."""
         if seeds is None:
             seeds = reference_seeds(MM_MAX_ROUNDS)
         o, keep = _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond,
-                             profile, graph_mode)
+                             profile, graph_mode, unroll)
         st = _Stats() if stats else None
         out = np.empty(self.n, np.int32) if fetch else None
         _check(lib().mm_match(self._h, ctypes.byref(o), _ptr(out), ctypes.byref(st) if stats else None))
@@ -388,11 +391,12 @@ class Batch:
 
     def match(self, kind: int = MM_WEIGHTED, maximal: bool = True, seeds=None, max_rounds: int = 0,
               barrier: int = MM_DEFAULT_BARRIER, block_size: int = 0, lanes: int = 0, hub_degree: int = 0,
-              respond: int = MM_RESPOND_AUTO, profile: bool = False, fetch: bool = True, graph_mode: int = 0):
+              respond: int = MM_RESPOND_AUTO, profile: bool = False, fetch: bool = True, graph_mode: int = 0,
+              unroll: int = 0):
         if seeds is None:
             seeds = reference_seeds(MM_MAX_ROUNDS)
         o, keep = _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond,
-                             profile, graph_mode)
+                             profile, graph_mode, unroll)
         st = _Stats()
         out = np.empty(self.total_n, np.int32) if fetch else None
         rounds = np.zeros(len(self.graphs), np.int32)

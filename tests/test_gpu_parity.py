@@ -55,8 +55,8 @@ def test_golden_tiny_cases_all_types_and_tiles():
             g = Graph(csr, keep_order=keep)
             modes = [MM_RESPOND_PULL] + ([MM_RESPOND_PUSH] if g.info()["push_ok"] else [])
             for respond in modes:
-                for lanes, block, hub in ((0, 256, 256), (0, 64, 3), (0, 1024, 1), (1, 64, 2), (4, 256, 256),
-                                          (16, 256, 3), (64, 128, 1)):
+                for lanes, block, hub in ((-1, 256, 256), (-1, 64, 3), (-1, 1024, 1), (0, 256, 256), (1, 64, 2),
+                                          (4, 256, 256), (16, 256, 3), (64, 128, 1)):
                     res = g.match(kind=MM_WEIGHTED if c["weighted"] else MM_RANDOM, maximal=c["maximal"],
                                   seeds=seeds, max_rounds=c["max_rounds"], barrier=c["barrier"], lanes=lanes,
                                   block_size=block, hub_degree=hub, respond=respond)
@@ -117,7 +117,7 @@ def test_shuffled_rows_use_exact_pull_path(keep):
     g = Graph(csr, keep_order=keep)
     assert g.info()["push_ok"] == 0
     for weighted in (False, True):
-        for lanes in (0, 8):
+        for lanes in (-1, 0, 2):
             res = check_against_oracle(csr, g, weighted, True, lanes=lanes)
             assert res.respond == MM_RESPOND_PULL
 
@@ -127,7 +127,7 @@ def test_tile_choices_do_not_change_results():
     csr = mmamd.gen_rgg(1 << 15, seed=8, weighted=True)
     g = Graph(csr)
     ref = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS)
-    for lanes in (0, 1, 2, 4, 8, 16, 32, 64):
+    for lanes in (-1, 0, 1, 2, 4, 8, 16, 32, 64):
         for block in (64, 256, 1024):
             for hub in (4, 12, 256):
                 res = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS, lanes=lanes, block_size=block,
@@ -143,7 +143,7 @@ def test_rmat_hub_path_bitexact(hub, keep):
     assert g.info()["max_degree"] > 256
     for weighted in (False, True):
         for respond in (MM_RESPOND_PULL, MM_RESPOND_PUSH):
-            for lanes in (0, 32):
+            for lanes in (-1, 0, 32):
                 check_against_oracle(csr, g, weighted, True, hub_degree=hub, respond=respond, lanes=lanes)
 
 
@@ -185,6 +185,8 @@ def test_empty_graph_and_bad_arguments():
         g.match(maximal=True, seeds=SEEDS[:2])  # cannot converge in 2 rounds
     with pytest.raises(mmamd.MatchError):
         g.match(seeds=SEEDS, lanes=3)
+    with pytest.raises(mmamd.MatchError):
+        g.match(seeds=SEEDS, lanes=-2)
     with pytest.raises(mmamd.MatchError):
         g.match(seeds=SEEDS, block_size=100)
     # a failed call leaves the handle usable
