@@ -59,8 +59,8 @@ struct Dev {
     int32_t bcap;         // capacity of one bucket; bucket b owns ids [b*bcap, (b+1)*bcap) in round 1
     const int2 *ranges;
     const int32_t *nbrs;  // unweighted layout
-    const int32_t *sid;   // weighted rows sorted for matching: neighbour ids, ~id for weights that never win
-    const float *sw;      // their weights
+    const int2 *adj;      // weighted rows sorted for matching: {neighbour id (~id when the
+                          // weight can never win), weight bits}
     int32_t *match;
     int32_t *req;
     uint8_t *st;
@@ -155,15 +155,18 @@ __device__ __forceinline__ void wave_append(bool pred, int4 value, int4 *list, i
 // once per round: non-temporal loads, so they do not evict the gathered
 // state arrays from L2.
 template <bool A8>
-__device__ __forceinline__ void load_id(const Dev &d, int32_t j, int32_t &u, bool &valid)
+__device__ __forceinline__ void load_id(const Dev &d, int32_t j, int32_t &u, bool &valid, float &w)
 {
     if (A8) {
-        const int32_t raw = __builtin_nontemporal_load(d.sid + j);
-        valid = raw >= 0;
-        u = valid ? raw : ~raw;
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        const v2i e = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(d.adj + j));
+        valid = e.x >= 0;
+        u = valid ? e.x : ~e.x;
+        w = valid ? __int_as_float(e.y) : -2.0f;
     } else {
         u = __builtin_nontemporal_load(d.nbrs + j);
         valid = true;
+        w = 1.0f;
     }
 }
 
@@ -171,8 +174,7 @@ template <bool A8>
 __device__ __forceinline__ void load_entry(const Dev &d, int32_t j, int32_t &u, float &w)
 {
     bool valid;
-    load_id<A8>(d, j, u, valid);
-    w = A8 ? (valid ? __builtin_nontemporal_load(d.sw + j) : -2.0f) : 1.0f;
+    load_id<A8>(d, j, u, valid, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -251,24 +253,27 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
         if (first + step + gw < count) nxt = load_ent(list + first + step + gw);
         const int32_t v = ent.x;
         const int2 rg = valid ? make_int2(ent.y, ent.z) : make_int2(0, 0);
-        int32_t win_u = -1, win_j = -1;
+        int32_t win_u = -1;
+        float win_w = 1.0f;
         bool alive = false, done = rg.x >= rg.y;
         int32_t base = rg.x;
         while (__ballot(!done) != 0ull) {
             const int32_t j = base + gl;
             int32_t u = 0;
+            float w = 1.0f;
             bool cand = false;
             if (!done && j < rg.y) {
                 bool ok;
-                load_id<A8>(d, j, u, ok);
+                load_id<A8>(d, j, u, ok, w);
                 cand = first_candidate<W, RESPOND>(d, v, u, ok, alive);
             }
             const unsigned long long m = __ballot(cand) & gmask;
             const int src = m ? __ffsll((long long)m) - 1 : lane;
             const int32_t su = __shfl(u, src);
+            const float sw = __shfl(w, src);
             if (!done && m) {
                 win_u = su;
-                win_j = base + (src & (G - 1));
+                win_w = sw;
                 done = true;
             }
             base += G;
@@ -276,8 +281,6 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
         }
         alive = (__ballot(alive) & gmask) != 0ull;
         if (valid && gl == 0) {
-            // Only a posted proposal needs the winner's weight.
-            const float win_w = (W && A8 && PUSH && !RESPOND && win_u >= 0) ? d.sw[win_j] : 1.0f;
             first_write<W, PUSH, RESPOND>(d, v, ent.w, win_u, win_w, alive, resp);
             nnz += (unsigned long long)(rg.y - rg.x);
         }
@@ -304,8 +307,9 @@ __device__ void first_scan_hubs(const Dev &d, const int4 *list, int32_t count, i
             const int32_t j = base + threadIdx.x;
             if (j < rg.y) {
                 int32_t u;
+                float w;
                 bool ok, alive = false;
-                load_id<A8>(d, j, u, ok);
+                load_id<A8>(d, j, u, ok, w);
                 if (first_candidate<W, RESPOND>(d, v, u, ok, alive)) atomicMin(&s_first, j);
                 if (alive) s_alive = 1;
             }
@@ -944,8 +948,7 @@ struct mm_graph {
     int64_t deg_hist[33] = {0};  // deg_hist[b]: vertices with degree in [2^b, 2^(b+1)) (b=0: deg 1)
     int2 *d_ranges = nullptr;
     int32_t *d_nbrs = nullptr;
-    int32_t *d_sid = nullptr;
-    float *d_sw = nullptr;
+    int2 *d_adj = nullptr;
     int32_t *d_match = nullptr;
     int32_t *d_req = nullptr;
     uint8_t *d_st = nullptr;
@@ -1011,7 +1014,7 @@ void release(mm_graph *g)
     if (!g) return;
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
-    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_sid,    g->d_sw,    g->d_match, g->d_req,   g->d_st,
+    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_match, g->d_req,   g->d_st,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
                     g->d_new_of, g->d_match_out};
@@ -1145,9 +1148,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     const size_t nn = (size_t)n;
     if ((rc = dev_alloc(g, &g->d_ranges, nn, "neighbour ranges")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_nbrs, (size_t)nnz, "neighbours")) != MM_OK) return fail(rc);
-    if (g->weighted && ((rc = dev_alloc(g, &g->d_sid, (size_t)nnz, "sorted rows")) != MM_OK ||
-                        (rc = dev_alloc(g, &g->d_sw, (size_t)nnz, "sorted weights")) != MM_OK))
-        return fail(rc);
+    if (g->weighted && (rc = dev_alloc(g, &g->d_adj, (size_t)nnz, "sorted weighted rows")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_match, nn, "match")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_req, nn, "requests")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_st, nn, "state")) != MM_OK) return fail(rc);
@@ -1197,12 +1198,8 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         off += deg;
     }
     std::vector<int32_t> h_nbrs((size_t)nnz);
-    std::vector<int32_t> h_sid;
-    std::vector<float> h_sw;
-    if (g->weighted) {
-        h_sid.resize((size_t)nnz);
-        h_sw.resize((size_t)nnz);
-    }
+    std::vector<int2> h_adj;
+    if (g->weighted) h_adj.resize((size_t)nnz);
     mm::parallel_for(n, [&](int64_t b, int64_t e, unsigned) {
         std::vector<int32_t> perm;
         for (int64_t i = b; i < e; ++i) {
@@ -1226,8 +1223,9 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
             for (int32_t k = 0; k < deg; ++k) {
                 const float wk = wr[perm[(size_t)k]];
                 const int32_t id = h_nbrs[(size_t)dst + perm[(size_t)k]];
-                h_sid[(size_t)dst + k] = wk > -1.0f ? id : ~id;
-                h_sw[(size_t)dst + k] = wk;
+                int32_t wb;
+                std::memcpy(&wb, &wk, sizeof(wb));
+                h_adj[(size_t)dst + k] = make_int2(wk > -1.0f ? id : ~id, wb);
             }
         }
     });
@@ -1237,8 +1235,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     if (nnz > 0) {
         if (hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
             (g->weighted &&
-             (hipMemcpy(g->d_sid, h_sid.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
-              hipMemcpy(g->d_sw, h_sw.data(), sizeof(float) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess)))
+             hipMemcpy(g->d_adj, h_adj.data(), sizeof(int2) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess))
             return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
     }
     if (g->relabelled) {
@@ -1425,8 +1422,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.bcap = c.bcap;
     d.ranges = g->d_ranges;
     d.nbrs = g->d_nbrs;
-    d.sid = g->d_sid;
-    d.sw = g->d_sw;
+    d.adj = g->d_adj;
     d.match = g->d_match;
     d.req = g->d_req;
     d.st = g->d_st;
