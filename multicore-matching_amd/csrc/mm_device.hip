@@ -35,6 +35,8 @@
 #include <vector>
 
 #include "../../include/mm_match.h"
+#include "mm_graph_impl.h"
+#include "mm_hash.h"
 #include "mm_internal.h"
 
 namespace {
@@ -103,33 +105,7 @@ __device__ __forceinline__ int bucket_of_block(const Dev &d)
     return (i & 7) * (d.nb >> 3) + (i >> 3);
 }
 
-__device__ __forceinline__ uint32_t rotl32(uint32_t x, uint32_t s) { return (x << s) | (x >> (32u - s)); }
-
-// The select hash of src/matchgpu.cu:196-217: MD5-shaped, 16 steps, SHA-256
-// constants K[0..15], MD5 round-1 rotations, per-step accumulation into h0..h3
-// and g multiplied by the seed after every step.
-__device__ __forceinline__ uint32_t select_hash(uint32_t i, uint32_t seed)
-{
-    uint32_t h0 = 0x67452301u, h1 = 0xefcdab89u, h2 = 0x98badcfeu, h3 = 0x10325476u;
-    uint32_t a = h0, b = h1, c = h2, d = h3, g = i;
-#define MM_STEP(K, R)                                         \
-    {                                                         \
-        const uint32_t f = (b & c) | (~b & d);                \
-        const uint32_t t = d;                                 \
-        d = c;                                                \
-        c = b;                                                \
-        b = b + rotl32(a + f + (K) + g, (R));                 \
-        a = t;                                                \
-        h0 += a; h1 += b; h2 += c; h3 += d;                   \
-        g *= seed;                                            \
-    }
-    MM_STEP(0x428a2f98u, 7) MM_STEP(0x71374491u, 12) MM_STEP(0xb5c0fbcfu, 17) MM_STEP(0xe9b5dba5u, 22)
-    MM_STEP(0x3956c25bu, 7) MM_STEP(0x59f111f1u, 12) MM_STEP(0x923f82a4u, 17) MM_STEP(0xab1c5ed5u, 22)
-    MM_STEP(0xd807aa98u, 7) MM_STEP(0x12835b01u, 12) MM_STEP(0x243185beu, 17) MM_STEP(0x550c7dc3u, 22)
-    MM_STEP(0x72be5d74u, 7) MM_STEP(0x80deb1feu, 12) MM_STEP(0x9bdc06a7u, 17) MM_STEP(0xc19bf174u, 22)
-#undef MM_STEP
-    return h0 + h1 + h2 + h3;
-}
+__device__ __forceinline__ uint32_t select_hash(uint32_t i, uint32_t seed) { return mm_select_hash(i, seed); }
 
 __device__ __forceinline__ int colour_of(uint32_t local_id, uint32_t seed, uint32_t barrier)
 {
@@ -924,66 +900,6 @@ __global__ void k_graph_rounds(const uint8_t *vround, const int32_t *voff, int32
 // Host side
 // ---------------------------------------------------------------------------
 
-// Key of a captured launch sequence: every value baked into its kernel arguments.
-struct ExecKey {
-    int32_t R_run, W, A8, PUSH, G, U, block, hub_deg, nb, bcap, maximal, hub_blocks, count_nnz, r0, r1;
-    uint32_t barrier;
-};
-struct CachedExec {
-    ExecKey key;
-    hipGraphExec_t exec;
-};
-
-struct mm_graph {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    int num_cu = 256;
-    int32_t n = 0;
-    int64_t nnz = 0;
-    int weighted = 0;
-    int push_ok_struct = 0;   // symmetric with strictly ascending rows
-    int push_ok_weights = 0;  // and w(u,v) == w(v,u)
-    int32_t max_degree = 0;
-    double mean_degree = 0.0;
-    int64_t deg_hist[33] = {0};  // deg_hist[b]: vertices with degree in [2^b, 2^(b+1)) (b=0: deg 1)
-    int2 *d_ranges = nullptr;
-    int32_t *d_nbrs = nullptr;
-    int2 *d_adj = nullptr;
-    int32_t *d_match = nullptr;
-    int32_t *d_req = nullptr;
-    uint8_t *d_st = nullptr;
-    unsigned long long *d_resp = nullptr;
-    int4 *d_lists = nullptr;
-    int32_t *d_ctr = nullptr;
-    int32_t *d_ctrb = nullptr;
-    unsigned long long *d_ctr64 = nullptr;
-    unsigned long long *d_ctrb64 = nullptr;
-    unsigned long long *d_hub64 = nullptr;
-    int32_t nb_max = 1;       // bucket slots allocated per counter row
-    int64_t list_cap = 0;     // capacity of one work list
-    uint32_t *d_seeds = nullptr;
-    int32_t *h_ctr = nullptr;               // pinned
-    unsigned long long *h_ctr64 = nullptr;  // pinned
-    // batched graphs
-    int32_t n_graphs = 0;
-    int32_t *d_loc = nullptr;        // graph-local original id per internal vertex
-    int32_t *d_new_of = nullptr;     // internal id per original (global) id, when relabelled
-    int32_t *d_match_out = nullptr;  // result in the caller's numbering, when relabelled
-    bool relabelled = false;
-    int32_t *d_voff = nullptr;
-    uint8_t *d_vround = nullptr;
-    int32_t *d_grounds = nullptr;
-    std::vector<uint32_t> last_seeds;  // seed table currently on the device
-    std::vector<CachedExec> execs;     // captured launch sequences
-    hipEvent_t ev_t1 = nullptr, ev_t2 = nullptr;
-    std::vector<hipEvent_t> prof_ev;
-    int64_t device_bytes = 0;
-};
-
-struct mm_batch {
-    mm_graph *g = nullptr;
-    std::vector<int32_t> vertex_offsets;
-};
 
 namespace {
 
@@ -1017,11 +933,13 @@ void release(mm_graph *g)
     void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_match, g->d_req,   g->d_st,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
-                    g->d_new_of, g->d_match_out};
+                    g->d_new_of, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_params,
+                    g->d_lsamp};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (g->h_ctr) hipHostFree(g->h_ctr);
     if (g->h_ctr64) hipHostFree(g->h_ctr64);
+    if (g->h_lctr) hipHostFree(g->h_lctr);
     if (g->ev_t1) hipEventDestroy(g->ev_t1);
     if (g->ev_t2) hipEventDestroy(g->ev_t2);
     for (hipEvent_t e : g->prof_ev) hipEventDestroy(e);
@@ -1562,9 +1480,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     if (g->n == 0) rounds = maximal ? 1 : R_max;
     if (rounds < 0) rounds = R_max;
     // The result in the caller's numbering is part of the matching.
-    if (g->relabelled && g->n > 0)
-        hipLaunchKernelGGL(k_unpermute, dim3(std::min<int64_t>(((int64_t)g->n + 255) / 256, (int64_t)g->num_cu * 8)),
-                           dim3(256), 0, g->stream, g->d_match, g->d_new_of, g->d_match_out, g->n);
+    mm_graph_enqueue_unpermute(g);
     MM_HIP_TRY(hipEventRecord(g->ev_t2, g->stream), "hipEventRecord");
     // Totals of every executed round (a maximal run's last, empty round has none).
     const int r_stats = g->n > 0 ? std::min(rounds, R_run) : 0;
@@ -1625,6 +1541,21 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
 }
 
 }  // namespace
+
+int32_t mm_graph_alloc(mm_graph *g, void **p, size_t bytes, const char *what)
+{
+    char *q = nullptr;
+    const int32_t rc = dev_alloc(g, &q, bytes, what);
+    *p = q;
+    return rc;
+}
+
+void mm_graph_enqueue_unpermute(mm_graph *g)
+{
+    if (g->relabelled && g->n > 0)
+        hipLaunchKernelGGL(k_unpermute, dim3(std::min<int64_t>(((int64_t)g->n + 255) / 256, (int64_t)g->num_cu * 8)),
+                           dim3(256), 0, g->stream, g->d_match, g->d_new_of, g->d_match_out, g->n);
+}
 
 extern "C" {
 

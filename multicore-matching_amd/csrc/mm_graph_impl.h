@@ -1,0 +1,88 @@
+// The device-side state of one uploaded graph (or batch of graphs), shared by
+// the kernel sources of the matcher library.
+#ifndef MM_GRAPH_IMPL_H
+#define MM_GRAPH_IMPL_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/mm_match.h"
+
+// Key of a captured launch sequence: every value baked into its kernel arguments.
+struct ExecKey {
+    int32_t R_run, W, A8, PUSH, G, U, block, hub_deg, nb, bcap, maximal, hub_blocks, count_nnz, r0, r1;
+    uint32_t barrier;
+};
+struct CachedExec {
+    ExecKey key;
+    hipGraphExec_t exec;
+};
+
+struct mm_graph {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 256;
+    int32_t n = 0;
+    int64_t nnz = 0;
+    int weighted = 0;
+    int push_ok_struct = 0;   // symmetric with strictly ascending rows
+    int push_ok_weights = 0;  // and w(u,v) == w(v,u)
+    int32_t max_degree = 0;
+    double mean_degree = 0.0;
+    int64_t deg_hist[33] = {0};  // deg_hist[b]: vertices with degree in [2^b, 2^(b+1)) (b=0: deg 1)
+    int2 *d_ranges = nullptr;
+    int32_t *d_nbrs = nullptr;
+    int2 *d_adj = nullptr;
+    int32_t *d_match = nullptr;
+    int32_t *d_req = nullptr;
+    uint8_t *d_st = nullptr;
+    unsigned long long *d_resp = nullptr;
+    int4 *d_lists = nullptr;
+    int32_t *d_ctr = nullptr;
+    int32_t *d_ctrb = nullptr;
+    unsigned long long *d_ctr64 = nullptr;
+    unsigned long long *d_ctrb64 = nullptr;
+    unsigned long long *d_hub64 = nullptr;
+    int32_t nb_max = 1;       // bucket slots allocated per counter row
+    int64_t list_cap = 0;     // capacity of one work list
+    uint32_t *d_seeds = nullptr;
+    int32_t *h_ctr = nullptr;               // pinned
+    unsigned long long *h_ctr64 = nullptr;  // pinned
+    // batched graphs
+    int32_t n_graphs = 0;
+    int32_t *d_loc = nullptr;        // graph-local original id per internal vertex
+    int32_t *d_new_of = nullptr;     // internal id per original (global) id, when relabelled
+    int32_t *d_match_out = nullptr;  // result in the caller's numbering, when relabelled
+    bool relabelled = false;
+    int32_t *d_voff = nullptr;
+    uint8_t *d_vround = nullptr;
+    int32_t *d_grounds = nullptr;
+    std::vector<uint32_t> last_seeds;  // seed table currently on the device
+    std::vector<CachedExec> execs;     // captured launch sequences
+    hipEvent_t ev_t1 = nullptr, ev_t2 = nullptr;
+    std::vector<hipEvent_t> prof_ev;
+    int64_t device_bytes = 0;
+    // learned selection (allocated on first use)
+    uint8_t *d_lcol = nullptr;      // colour chosen by the learned select: 0 blue, 1 red, 2 no unmatched neighbour
+    float *d_lreqw = nullptr;       // weight of the edge a vertex proposed along or accepted
+    float *d_lfeat = nullptr;       // [n][8] features of the current round (traces only)
+    int32_t *d_lctr = nullptr;      // [MM_MAX_ROUNDS + 2][4] per-round counters of the learned rounds
+    float *d_params = nullptr;      // network parameters
+    unsigned long long *d_lsamp = nullptr;  // [2] samples written / offered
+    int32_t *h_lctr = nullptr;      // pinned
+};
+
+struct mm_batch {
+    mm_graph *g = nullptr;
+    std::vector<int32_t> vertex_offsets;
+};
+
+// Device allocation charged to the graph; MM_OK or MM_ERR_NOMEM (message set).
+int32_t mm_graph_alloc(mm_graph *g, void **p, size_t bytes, const char *what);
+// Enqueues the copy of the internal match array into the caller's vertex order
+// (relabelled graphs only; a no-op otherwise).
+void mm_graph_enqueue_unpermute(mm_graph *g);
+
+#endif

@@ -1,4 +1,5 @@
-"""ctypes binding of libmmamd.so, the MI355X greedy matcher's C ABI (include/mm_match.h).
+"""ctypes binding of libmmamd.so, the MI355X greedy matcher's C ABI (include/mm_match.h,
+include/mm_learn.h).
 
 Python is only the test/bench host here: it hands numpy arrays to the C ABI and
 reads results back.  The matcher itself runs in the HIP kernels of
@@ -10,6 +11,7 @@ Mirrors the reference's operator surface (BasFaggingerAuer/Multicore-Matching):
   Graph.match(...)       <- GraphMatchingGPU*::performMatching (src/matchgpu.cu:453-851)
   TYPE_* codes           <- bin/match.cpp:145-148 (types 6-9)
   reference_seeds(...)   <- srand(12345) then one rand() per round (bin/match.cpp:382, src/matchgpu.cu:509)
+  Graph.match_learned    <- no counterpart: network-chosen blue probabilities (include/mm_learn.h)
 """
 from __future__ import annotations
 
@@ -104,7 +106,40 @@ class _Csr(ctypes.Structure):
     ]
 
 
-# Every symbol include/mm_match.h declares, with its ctypes signature.
+MM_MLP_FEATURES, MM_MLP_HIDDEN, MM_MLP_PARAMS = 8, 32, 1377
+MLP_OFF = {"W1": 0, "b1": 256, "W2": 288, "b2": 1312, "W3": 1344, "b3": 1376}
+MM_LEARN_DRAW_TWEAK, MM_LEARN_TRACE_TWEAK = 0x5BD1E995, 0x2545F491
+
+
+class _LearnOpts(ctypes.Structure):
+    _fields_ = [
+        ("params", ctypes.c_void_p),
+        ("trace", ctypes.c_int32),
+        ("trace_keep", ctypes.c_uint32),
+        ("trace_cap", ctypes.c_int64),
+        ("trace_x", ctypes.c_void_p),
+        ("trace_y", ctypes.c_void_p),
+        ("trace_w", ctypes.c_void_p),
+        ("trace_v", ctypes.c_void_p),
+        ("trace_round", ctypes.c_void_p),
+        ("trace_count", ctypes.c_int64),
+        ("trace_offered", ctypes.c_int64),
+    ]
+
+
+class _TrainOpts(ctypes.Structure):
+    _fields_ = [
+        ("epochs", ctypes.c_int32),
+        ("batch", ctypes.c_int32),
+        ("lr", ctypes.c_float),
+        ("beta1", ctypes.c_float),
+        ("beta2", ctypes.c_float),
+        ("eps", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+# Every symbol include/*.h declares, with its ctypes signature.
 _P = ctypes.c_void_p
 _I32, _I64, _U64, _D = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
 SYMBOLS = {
@@ -125,6 +160,11 @@ SYMBOLS = {
     "mm_csr_free": (None, [ctypes.POINTER(_Csr)]),
     "mm_get_weight": (_I32, [_I32, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
     "mm_test_matching": (_I32, [_I32, _P, _P, _P, _P]),
+    "mm_match_learned": (_I32, [_P, ctypes.POINTER(_Opts), ctypes.POINTER(_LearnOpts), _P, ctypes.POINTER(_Stats)]),
+    "mm_mlp_init": (_I32, [_U64, _P]),
+    "mm_mlp_forward": (_I32, [_I32, _P, _P, _I64, _P]),
+    "mm_mlp_loss_grad": (_I32, [_I32, _P, _P, _P, _P, _I64, ctypes.POINTER(_D), _P]),
+    "mm_mlp_train": (_I32, [_I32, _P, _P, _P, _I64, ctypes.POINTER(_TrainOpts), _P, _P]),
 }
 
 _lib = None
@@ -345,6 +385,41 @@ class Graph:
         del keep
         return _result(st, out) if stats else None
 
+    def match_learned(self, params=None, kind: int = MM_WEIGHTED, maximal: bool = True, seeds=None,
+                      max_rounds: int = 0, barrier: int = MM_DEFAULT_BARRIER, block_size: int = 0,
+                      fetch: bool = True, trace: bool = False, trace_keep: int = 65536, trace_cap: int | None = None):
+        """Rounds with network-chosen blue probabilities (mm_match_learned);
+        params=None runs the plain colouring through the same kernels.  With
+        trace=True returns (result, samples) where samples is a dict of x, y, w,
+        v, round arrays ordered by (round, v) and 'offered'."""
+        if seeds is None:
+            seeds = reference_seeds(MM_MAX_ROUNDS)
+        o, keep = _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, 0, 0, 0, False)
+        lo = _LearnOpts()
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        if p is not None and p.shape != (MM_MLP_PARAMS,):
+            raise ValueError(f"params must hold {MM_MLP_PARAMS} floats")
+        lo.params = None if p is None else p.ctypes.data
+        bufs = None
+        if trace:
+            cap = self.n * 4 if trace_cap is None else int(trace_cap)
+            bufs = {"x": np.zeros((cap, MM_MLP_FEATURES), np.float32), "y": np.zeros(cap, np.float32),
+                    "w": np.zeros(cap, np.float32), "v": np.zeros(cap, np.int32), "round": np.zeros(cap, np.int32)}
+            lo.trace, lo.trace_keep, lo.trace_cap = 1, trace_keep, cap
+            lo.trace_x, lo.trace_y, lo.trace_w = bufs["x"].ctypes.data, bufs["y"].ctypes.data, bufs["w"].ctypes.data
+            lo.trace_v, lo.trace_round = bufs["v"].ctypes.data, bufs["round"].ctypes.data
+        st = _Stats()
+        out = np.empty(self.n, np.int32) if fetch else None
+        _check(lib().mm_match_learned(self._h, ctypes.byref(o), ctypes.byref(lo), _ptr(out), ctypes.byref(st)))
+        del keep
+        res = _result(st, out)
+        if not trace:
+            return res
+        k = int(lo.trace_count)
+        samples = {key: a[:k] for key, a in bufs.items()}
+        samples["offered"] = int(lo.trace_offered)
+        return res, samples
+
     def match_type(self, type_code: int, seeds=None, **kw) -> MatchResult:
         kind, maximal = TYPE_TO_KIND[type_code]
         return self.match(kind=kind, maximal=bool(maximal), seeds=seeds, **kw)
@@ -418,3 +493,44 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """He-uniform weights, zero biases (mm_mlp_init)."""
+    p = np.zeros(MM_MLP_PARAMS, np.float32)
+    _check(lib().mm_mlp_init(seed, _ptr(p)))
+    return p
+
+
+def mlp_forward(params, x, device: int = 0) -> np.ndarray:
+    """Network outputs z of feature rows x[n, 8] on the device (mm_mlp_forward)."""
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MM_MLP_FEATURES)
+    z = np.zeros(len(x), np.float32)
+    _check(lib().mm_mlp_forward(device, _ptr(p), _ptr(x), len(x), _ptr(z)))
+    return z
+
+
+def mlp_loss_grad(params, x, y, w, device: int = 0):
+    """(loss, gradient) of (1/n) sum w (softplus(z) - y z) by the device kernels."""
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MM_MLP_FEATURES)
+    y = np.ascontiguousarray(y, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    g = np.zeros(MM_MLP_PARAMS, np.float32)
+    loss = ctypes.c_double()
+    _check(lib().mm_mlp_loss_grad(device, _ptr(p), _ptr(x), _ptr(y), _ptr(w), len(x), ctypes.byref(loss), _ptr(g)))
+    return float(loss.value), g
+
+
+def mlp_train(x, y, w, params, epochs: int = 4, batch: int = 4096, lr: float = 3e-3, seed: int = 1,
+              device: int = 0):
+    """Minibatch Adam on the device (mm_mlp_train): (trained params, per-epoch mean loss)."""
+    p = np.array(params, np.float32, copy=True)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MM_MLP_FEATURES)
+    y = np.ascontiguousarray(y, np.float32)
+    w = np.ascontiguousarray(w, np.float32)
+    t = _TrainOpts(epochs, batch, lr, 0.9, 0.999, 1e-8, seed)
+    el = np.zeros(epochs, np.float32)
+    _check(lib().mm_mlp_train(device, _ptr(x), _ptr(y), _ptr(w), len(x), ctypes.byref(t), _ptr(p), _ptr(el)))
+    return p, el

@@ -49,6 +49,14 @@ def lib():
         h.orc_cpu_weighted.argtypes = [ctypes.c_int32, _P, _P, _P, _P]
         h.orc_cpu_weighted_edge.restype = None
         h.orc_cpu_weighted_edge.argtypes = [ctypes.c_int32, ctypes.c_int64, _P, _P, _P, _P]
+        h.orc_mlp_z_many.restype = None
+        h.orc_mlp_z_many.argtypes = [_P, _P, ctypes.c_int64, _P]
+        h.orc_blue_threshold_many.restype = None
+        h.orc_blue_threshold_many.argtypes = [_P, ctypes.c_int64, _P]
+        h.orc_match_learned.restype = ctypes.c_int32
+        h.orc_match_learned.argtypes = [ctypes.c_int32, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_uint32, _P, ctypes.c_int32, _P, _P, _P, _P, ctypes.c_uint32,
+                                        ctypes.c_int64, _P, _P, _P, _P, _P, _P]
         _lib = h
     return _lib
 
@@ -146,3 +154,95 @@ def cpu_weighted_edge(csr) -> np.ndarray:
     out = np.empty(csr.n, np.int32)
     lib().orc_cpu_weighted_edge(csr.n, len(ex), _p(ex), _p(ey), _p(ew), _p(out))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Learned selection (restates include/mm_learn.h; no reference counterpart)
+
+NF, NH, NP = 8, 32, 1377
+
+
+def mlp_z(params, x) -> np.ndarray:
+    """Network outputs with the specified float32 fused multiply-add order."""
+    p = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, NF)
+    z = np.empty(len(x), np.float32)
+    lib().orc_mlp_z_many(_p(p), _p(x), len(x), _p(z))
+    return z
+
+
+def blue_threshold(z) -> np.ndarray:
+    z = np.ascontiguousarray(z, np.float32)
+    out = np.empty(z.shape, np.uint32)
+    lib().orc_blue_threshold_many(_p(z), z.size, _p(out))
+    return out
+
+
+def match_learned(csr, weighted: bool, maximal: bool, seeds, params=None, max_rounds: int = 0,
+                  barrier: int = 0x88B81733, trace: bool = False, trace_keep: int = 65536, trace_cap: int = 0):
+    """Learned rounds; returns (match, rounds, active[], matched[], samples or None)."""
+    if max_rounds <= 0:
+        max_rounds = 255 if maximal else 20
+    ranges, nb, w = _csr_arrays(csr)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
+    p = None if params is None else np.ascontiguousarray(params, np.float32)
+    n = csr.n
+    out = np.empty(n, np.int32)
+    act = np.zeros(256, np.int64)
+    mat = np.zeros(256, np.int64)
+    cap = (trace_cap or 4 * max(n, 1)) if trace else 0
+    tx = np.zeros((max(cap, 1), NF), np.float32)
+    ty, tw = np.zeros(max(cap, 1), np.float32), np.zeros(max(cap, 1), np.float32)
+    tv, tr = np.zeros(max(cap, 1), np.int32), np.zeros(max(cap, 1), np.int32)
+    cnt = ctypes.c_int64()
+    R = lib().orc_match_learned(n, _p(ranges), _p(nb), _p(w), int(weighted), int(maximal), max_rounds, barrier,
+                                _p(seeds), len(seeds), _p(p), _p(out), _p(act), _p(mat), trace_keep, cap,
+                                _p(tx), _p(ty), _p(tw), _p(tv), _p(tr), ctypes.byref(cnt))
+    if R < 0:
+        raise ValueError("oracle: not enough seeds")
+    samples = None
+    if trace:
+        k = min(int(cnt.value), cap)
+        samples = {"x": tx[:k].copy(), "y": ty[:k].copy(), "w": tw[:k].copy(), "v": tv[:k].copy(),
+                   "round": tr[:k].copy(), "offered": int(cnt.value)}
+    return out, int(R), act[:R].copy(), mat[:R].copy(), samples
+
+
+def _unpack(params):
+    p = np.asarray(params, np.float64)
+    return (p[0:256].reshape(32, 8), p[256:288], p[288:1312].reshape(32, 32), p[1312:1344], p[1344:1376],
+            p[1376])
+
+
+def mlp_forward64(params, x):
+    """float64 forward pass: (z, cache)."""
+    W1, b1, W2, b2, W3, b3 = _unpack(params)
+    x = np.asarray(x, np.float64).reshape(-1, NF)
+    a1 = x @ W1.T + b1
+    h1 = np.maximum(a1, 0.0)
+    a2 = h1 @ W2.T + b2
+    h2 = np.maximum(a2, 0.0)
+    z = h2 @ W3 + b3
+    return z, (x, a1, h1, a2, h2)
+
+
+def mlp_loss_grad64(params, x, y, w):
+    """float64 loss (1/n) sum w (softplus(z) - y z) and its gradient, by
+    hand-derived backpropagation (checked against finite differences)."""
+    W1, b1, W2, b2, W3, b3 = _unpack(params)
+    z, (x, a1, h1, a2, h2) = mlp_forward64(params, x)
+    y = np.asarray(y, np.float64)
+    w = np.asarray(w, np.float64)
+    n = len(z)
+    loss = float(np.sum(w * (np.logaddexp(0.0, z) - y * z)) / n)
+    dz = w * (1.0 / (1.0 + np.exp(-z)) - y) / n
+    gW3 = h2.T @ dz
+    gb3 = dz.sum()
+    d2 = np.outer(dz, W3) * (a2 > 0)
+    gW2 = d2.T @ h1
+    gb2 = d2.sum(0)
+    d1 = (d2 @ W2) * (a1 > 0)
+    gW1 = d1.T @ x
+    gb1 = d1.sum(0)
+    g = np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3, [gb3]])
+    return loss, g
