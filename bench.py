@@ -12,7 +12,10 @@ convergence (the run is deterministic, so the round index is fixed).  value =
 m / t_99 / 1e9 with m = undirected edges.  The JSON line also carries the
 roofline of the dominant kernel (HIP events on the library's stream) and the
 CPU baseline (the OpenMP restatement of the reference's TBB round matcher,
-type 11, timed on this host).
+type 11, timed on this host), and the learned selection against the plain
+rounds on the same graph: rounds, matched vertices, weight and device time to
+convergence, and the rounds and time the learned run needs to reach the plain
+run's final weight.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--log2n 22]
 """
@@ -108,6 +111,9 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--respond", type=int, default=mmamd.MM_RESPOND_AUTO)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-selection comparison")
+    ap.add_argument("--params", default=os.path.join(ROOT, "multicore-matching_amd", "mlp_params.json"),
+                    help="network parameters of the learned selection (learn.py train)")
     ap.add_argument("--keep-order", action="store_true", help="no internal breadth-first renumbering")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run only a plain device copy of this many MiB (calibrates rocprofv3 byte counters)")
@@ -205,6 +211,13 @@ def main():
         "round_roofline_frac": round(kb["round_total"] / (t99_event_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
     }
+    if not args.no_learned:
+        import learn
+
+        cmp = learn.compare(csr, learn.load_params(args.params), seeds=seeds, graph=g)
+        cmp["params"] = os.path.relpath(args.params, ROOT)
+        line["learned_vs_plain"] = cmp
+        log(f"learned vs plain: {json.dumps(cmp)}")
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         line["cpu_baseline"] = cpu_baseline(csr, seeds, r99)

@@ -69,6 +69,7 @@ struct mm_graph {
     float *d_lreqw = nullptr;       // weight of the edge a vertex proposed along or accepted
     float *d_lfeat = nullptr;       // [n][8] features of the current round (traces only)
     int32_t *d_lctr = nullptr;      // [MM_MAX_ROUNDS + 2][4] per-round counters of the learned rounds
+    int32_t *d_lpart = nullptr;     // per-block partial counts of the current learned round
     float *d_params = nullptr;      // network parameters
     unsigned long long *d_lsamp = nullptr;  // [2] samples written / offered
     int32_t *h_lctr = nullptr;      // pinned

@@ -39,9 +39,12 @@ namespace {
 
 constexpr int NF = MM_MLP_FEATURES, NH = MM_MLP_HIDDEN, NP = MM_MLP_PARAMS;
 constexpr int LCTR = 4;  // per round: active at select, blue, newly matched, newly dead
+constexpr int LPART = 4096;  // most blocks of a counting launch
 constexpr int LSLOTS = MM_MAX_ROUNDS + 2;
 constexpr int32_t LREQ_NONE = -1, LREQ_DEAD = -2;
 constexpr uint8_t LC_BLUE = 0, LC_RED = 1, LC_RETIRE = 2;
+constexpr int SEL_G = 8;   // lanes per row in the select's feature scans
+constexpr int SCAN_G = 4;  // lanes per row in the request / respond scans
 
 static_assert(MM_MLP_OFF_B3 + 1 == NP, "parameter layout");
 
@@ -119,7 +122,7 @@ struct LDev {
     float *reqw;
     float *feat;
     int32_t *ctr;  // [LSLOTS][LCTR]
-    const float *params;
+    int32_t *part; // [LCTR][LPART] per-block partial counts of the current round
     float *sx, *sy, *sw;
     int32_t *sv, *sr;
     unsigned long long *scount;
@@ -142,12 +145,21 @@ __device__ __forceinline__ void l_entry(const LDev &d, int32_t j, int32_t &u, bo
     }
 }
 
-// Adds the number of lanes with pred to *ctr with one atomic per wavefront.
-// Every lane of the wavefront must reach it.
-__device__ __forceinline__ void wave_count(bool pred, int32_t *ctr)
+// Sum of x over the block (every thread must call it; blockDim.x a multiple
+// of 64, at most 1024); the result is valid in thread 0.
+__device__ __forceinline__ int block_sum(int x)
 {
-    const unsigned long long m = __ballot(pred);
-    if (m && __lane_id() == __ffsll((long long)m) - 1) atomicAdd(ctr, (int)__popcll(m));
+    __shared__ int ws[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    const int wid = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) ws[wid] = x;
+    __syncthreads();
+    int s = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += ws[k];
+    return s;
 }
 
 __global__ void k_l_fill(int32_t *p, int64_t count, int32_t value)
@@ -156,29 +168,59 @@ __global__ void k_l_fill(int32_t *p, int64_t count, int32_t value)
         p[i] = value;
 }
 
-template <bool A8>
-__global__ void k_l_select(LDev d, int r, uint32_t seed)
+constexpr int LTILE = 256;  // vertices per select tile = threads per select block
+
+// Sum over the G aligned lanes of a group (G a power of two <= 64).
+template <int G>
+__device__ __forceinline__ int group_sum(int x)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < d.n; base += stride) {
-        const int32_t v = (int32_t)(base + threadIdx.x);
-        const bool inb = v < d.n;
-        const int32_t m = inb ? d.match[v] : 4;
-        const bool act = m < 2;
-        int c = LC_RED;
-        if (act) {
-            const uint32_t vl = (uint32_t)d.loc[v];
-            const uint32_t hv = mm_select_hash(vl, seed);
-            const int plain = hv < d.barrier ? LC_BLUE : LC_RED;
-            const bool net = d.use_mlp && r <= MM_LEARN_ROUNDS;
-            if (!net && !d.trace) {
-                c = plain;
-            } else {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// The two smallest of the group's (p1 < p2) position pairs.
+template <int G>
+__device__ __forceinline__ void group_min2(int &p1, int &p2)
+{
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        const int q1 = __shfl_xor(p1, o), q2 = __shfl_xor(p2, o);
+        const int n1 = min(p1, q1);
+        p2 = min(max(p1, q1), min(p2, q2));
+        p1 = n1;
+    }
+}
+
+// Select of round r over tiles of LTILE vertices.  Phase 1: groups of G lanes
+// scan the rows of the tile's active vertices (coalesced chunks of G entries)
+// for the features, into LDS.  Phase 2: one thread per vertex evaluates the
+// network and draws the colour.  The network parameters arrive as their own
+// read-only, non-aliased argument, so their wave-uniform loads are scalar.
+template <bool A8, int G>
+__global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed, const float *__restrict__ params)
+{
+    __shared__ float fs[LTILE][NF + 1];
+    __shared__ int8_t fl[LTILE];  // -1 inactive, 0 active, 1 active without unmatched neighbours
+    const int t = threadIdx.x;
+    const int gl = t & (G - 1);
+    const int grp = t / G;
+    constexpr int GROUPS = LTILE / G;
+    const bool net = d.use_mlp && r <= MM_LEARN_ROUNDS;
+    const bool scan = net || d.trace;
+    int n_act = 0, n_blue = 0;
+    for (int64_t base = (int64_t)blockIdx.x * LTILE; base < d.n; base += (int64_t)gridDim.x * LTILE) {
+        if (scan) {
+            for (int k = grp; k < LTILE; k += GROUPS) {
+                const int64_t v = base + k;
+                const int32_t m = v < d.n ? d.match[v] : 4;
+                if (m >= 2) {
+                    if (gl == 0) fl[k] = -1;
+                    continue;
+                }
                 const int2 rg = d.ranges[v];
-                int32_t alive = 0, redp = 0, ustar = -1;
-                float w1 = 0.0f, w2 = 0.0f;
-                bool have2 = false;
-                for (int32_t j = rg.x; j < rg.y; ++j) {
+                int alive = 0, redp = 0, p1 = INT_MAX, p2 = INT_MAX;
+                for (int32_t j = rg.x + gl; j < rg.y; j += G) {
                     int32_t u;
                     bool valid;
                     float w;
@@ -187,97 +229,176 @@ __global__ void k_l_select(LDev d, int r, uint32_t seed)
                     if (mu >= 4) continue;
                     ++alive;
                     redp += mu == 1;
-                    if (!valid) continue;
-                    if (ustar < 0) {
-                        ustar = u;
-                        w1 = w;
-                    } else if (!have2) {
-                        w2 = w;
-                        have2 = true;
+                    if (valid) {
+                        if (p1 == INT_MAX) p1 = j;
+                        else if (p2 == INT_MAX) p2 = j;
                     }
                 }
+                alive = group_sum<G>(alive);
+                redp = group_sum<G>(redp);
+                group_min2<G>(p1, p2);
+                if (gl == 0) {
+                    float w1 = 0.0f, w2 = 0.0f, bit = 0.0f;
+                    if (p1 != INT_MAX) {
+                        int32_t u;
+                        bool valid;
+                        l_entry<A8>(d, p1, u, valid, w1);
+                        const uint32_t hv = mm_select_hash((uint32_t)d.loc[v], seed);
+                        bit = hv < mm_select_hash((uint32_t)d.loc[u], seed) ? 1.0f : 0.0f;
+                    }
+                    if (p2 != INT_MAX) {
+                        int32_t u;
+                        bool valid;
+                        l_entry<A8>(d, p2, u, valid, w2);
+                    }
+                    fs[k][0] = (float)min(rg.y - rg.x, 1024) * 0.015625f;
+                    fs[k][1] = w1;
+                    fs[k][2] = w2;
+                    fs[k][3] = w1 > 0.0f ? (float)((double)w2 / (double)w1) : 0.0f;
+                    fs[k][4] = (float)min(alive, 1024) * 0.015625f;
+                    fs[k][5] = (float)min(redp, 1024) * 0.015625f;
+                    fs[k][6] = (float)(r - 1) * 0.03125f;
+                    fs[k][7] = bit;
+                    fl[k] = alive == 0 ? 1 : 0;
+                }
+            }
+            __syncthreads();
+        }
+        const int64_t v = base + t;
+        const int32_t m = v < d.n ? d.match[v] : 4;
+        const bool act = m < 2;
+        int c = LC_RED;
+        if (act) {
+            const uint32_t vl = (uint32_t)d.loc[v];
+            const int plain = mm_select_hash(vl, seed) < d.barrier ? LC_BLUE : LC_RED;
+            if (!net) {
+                c = plain;
+            } else if (fl[t] == 1) {
+                c = LC_RETIRE;
+            } else {
                 float x[NF];
-                x[0] = (float)min(rg.y - rg.x, 1024) * 0.015625f;
-                x[1] = w1;
-                x[2] = w2;
-                x[3] = w1 > 0.0f ? (float)((double)w2 / (double)w1) : 0.0f;
-                x[4] = (float)min(alive, 1024) * 0.015625f;
-                x[5] = (float)min(redp, 1024) * 0.015625f;
-                x[6] = (float)(r - 1) * 0.03125f;
-                x[7] = (ustar >= 0 && hv < mm_select_hash((uint32_t)d.loc[ustar], seed)) ? 1.0f : 0.0f;
-                if (d.trace) {
 #pragma unroll
-                    for (int k = 0; k < NF; ++k) d.feat[(size_t)v * NF + k] = x[k];
-                }
-                if (!net) {
+                for (int k = 0; k < NF; ++k) x[k] = fs[t][k];
+                const float z = mlp_z(params, x);
+                if (!isfinite(z)) {
                     c = plain;
-                } else if (alive == 0) {
-                    c = LC_RETIRE;
                 } else {
-                    const float z = mlp_z(d.params, x);
-                    if (!isfinite(z)) {
-                        c = plain;
-                    } else {
-                        const uint32_t T = blue_threshold(z);
-                        c = (mm_select_hash(vl, seed ^ MM_LEARN_DRAW_TWEAK) >> 16) < T ? LC_BLUE : LC_RED;
-                    }
+                    const uint32_t T = blue_threshold(z);
+                    c = (mm_select_hash(vl, seed ^ MM_LEARN_DRAW_TWEAK) >> 16) < T ? LC_BLUE : LC_RED;
                 }
             }
             d.col[v] = (uint8_t)c;
-        }
-        wave_count(act, d.ctr + (size_t)r * LCTR + 0);
-        wave_count(act && c == LC_BLUE, d.ctr + (size_t)r * LCTR + 1);
-    }
-}
-
-template <bool A8>
-__global__ void k_l_request(LDev d)
-{
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < d.n; v += stride) {
-        if (d.match[v] >= 2) continue;
-        if (d.col[v] != LC_BLUE) {
-            d.req[v] = LREQ_NONE;
-            continue;
-        }
-        const int2 rg = d.ranges[v];
-        bool alive = false;
-        int32_t best = -1;
-        float bw = 0.0f;
-        for (int32_t j = rg.x; j < rg.y; ++j) {
-            int32_t u;
-            bool valid;
-            float w;
-            l_entry<A8>(d, j, u, valid, w);
-            const int32_t mu = d.match[u];
-            if (mu < 4) alive = true;
-            if (valid && mu < 2 && d.col[u] == LC_RED) {
-                best = u;
-                bw = w;
-                break;
+            if (d.trace) {
+#pragma unroll
+                for (int k = 0; k < NF; ++k) d.feat[(size_t)v * NF + k] = fs[t][k];
             }
         }
-        d.req[v] = best >= 0 ? best : (alive ? LREQ_NONE : LREQ_DEAD);
-        d.reqw[v] = bw;
+        n_act += act;
+        n_blue += act && c == LC_BLUE;
+        if (scan) __syncthreads();
+    }
+    n_act = block_sum(n_act);
+    n_blue = block_sum(n_blue);
+    if (threadIdx.x == 0) {
+        d.part[0 * LPART + blockIdx.x] = n_act;
+        d.part[1 * LPART + blockIdx.x] = n_blue;
     }
 }
 
-template <bool A8>
+// Bits of this lane's group in a wavefront ballot, as a G-bit mask.
+template <int G>
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long ballot)
+{
+    const int shift = (__lane_id() / G) * G;
+    return G == 64 ? ballot : (ballot >> shift) & ((1ull << G) - 1ull);
+}
+
+// Request: each blue vertex's group scans its row in matching order, G
+// entries at a time, and stops at the first red unmatched neighbour.
+template <bool A8, int G>
+__global__ void k_l_request(LDev d)
+{
+    const int gl = __lane_id() & (G - 1);
+    const int64_t gstride = (int64_t)gridDim.x * blockDim.x / G;
+    const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int64_t n_round = ((int64_t)d.n + gstride - 1) / gstride * gstride;
+    for (int64_t v = g0; v < n_round; v += gstride) {
+        bool blue = false;
+        if (v < d.n && d.match[v] < 2) {
+            if (d.col[v] == LC_BLUE) blue = true;
+            else if (gl == 0) d.req[v] = LREQ_NONE;
+        }
+        int2 rg = make_int2(0, 0);
+        if (blue) rg = d.ranges[v];
+        bool alive = false, found = false;
+        int32_t best = -1;
+        float bw = 0.0f;
+        for (int32_t jb = rg.x; __ballot(blue && !found && jb < rg.y) != 0ull; jb += G) {
+            const bool live = blue && !found && jb < rg.y;
+            const int32_t j = jb + gl;
+            bool cand = false;
+            int32_t u = 0;
+            float w = 0.0f;
+            if (live && j < rg.y) {
+                bool valid;
+                l_entry<A8>(d, j, u, valid, w);
+                const int32_t mu = d.match[u];
+                alive |= mu < 4;
+                cand = valid && mu < 2 && d.col[u] == LC_RED;
+            }
+            const unsigned long long gm = group_bits<G>(__ballot(cand));
+            const int src = (__lane_id() & ~(G - 1)) + (gm ? __ffsll((long long)gm) - 1 : 0);
+            const int32_t su = __shfl(u, src);
+            const float sw = __shfl(w, src);
+            if (live && gm) {
+                best = su;
+                bw = sw;
+                found = true;
+            }
+        }
+        const bool any_alive = group_bits<G>(__ballot(alive)) != 0ull;
+        if (blue && gl == 0) {
+            d.req[v] = found ? best : (any_alive ? LREQ_NONE : LREQ_DEAD);
+            d.reqw[v] = found ? bw : 0.0f;
+        }
+    }
+}
+
+// Respond: each red vertex's group scans its row in matching order for the
+// first blue unmatched neighbour that proposed to it.
+template <bool A8, int G>
 __global__ void k_l_respond(LDev d)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < d.n; v += stride) {
-        if (d.match[v] >= 2 || d.col[v] != LC_RED) continue;
-        const int2 rg = d.ranges[v];
-        for (int32_t j = rg.x; j < rg.y; ++j) {
-            int32_t u;
-            bool valid;
-            float w;
-            l_entry<A8>(d, j, u, valid, w);
-            if (valid && d.match[u] < 2 && d.col[u] == LC_BLUE && d.req[u] == (int32_t)v) {
-                d.req[v] = u;
-                d.reqw[v] = w;
-                break;
+    const int gl = __lane_id() & (G - 1);
+    const int64_t gstride = (int64_t)gridDim.x * blockDim.x / G;
+    const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int64_t n_round = ((int64_t)d.n + gstride - 1) / gstride * gstride;
+    for (int64_t v = g0; v < n_round; v += gstride) {
+        const bool red = v < d.n && d.match[v] < 2 && d.col[v] == LC_RED;
+        int2 rg = make_int2(0, 0);
+        if (red) rg = d.ranges[v];
+        bool found = false;
+        for (int32_t jb = rg.x; __ballot(red && !found && jb < rg.y) != 0ull; jb += G) {
+            const bool live = red && !found && jb < rg.y;
+            const int32_t j = jb + gl;
+            bool cand = false;
+            int32_t u = 0;
+            float w = 0.0f;
+            if (live && j < rg.y) {
+                bool valid;
+                l_entry<A8>(d, j, u, valid, w);
+                cand = valid && d.match[u] < 2 && d.col[u] == LC_BLUE && d.req[u] == (int32_t)v;
+            }
+            const unsigned long long gm = group_bits<G>(__ballot(cand));
+            const int src = (__lane_id() & ~(G - 1)) + (gm ? __ffsll((long long)gm) - 1 : 0);
+            const int32_t su = __shfl(u, src);
+            const float sw = __shfl(w, src);
+            if (live && gm) {
+                if (gl == 0) {
+                    d.req[v] = su;
+                    d.reqw[v] = sw;
+                }
+                found = true;
             }
         }
     }
@@ -286,6 +407,7 @@ __global__ void k_l_respond(LDev d)
 __global__ void k_l_match(LDev d, int r, uint32_t seed)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int n_mat = 0, n_dead = 0;
     for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < d.n; base += stride) {
         const int32_t v = (int32_t)(base + threadIdx.x);
         const bool inb = v < d.n;
@@ -303,8 +425,8 @@ __global__ void k_l_match(LDev d, int r, uint32_t seed)
             else nm = c;
             d.match[v] = nm;
         }
-        wave_count(act && nm >= 4, d.ctr + (size_t)r * LCTR + 2);
-        wave_count(act && nm == 2, d.ctr + (size_t)r * LCTR + 3);
+        n_mat += act && nm >= 4;
+        n_dead += act && nm == 2;
         if (d.trace) {
             const bool keep = act && nm >= 4 && (mm_select_hash(vl, seed ^ MM_LEARN_TRACE_TWEAK) >> 16) < d.trace_keep;
             const unsigned long long mask = __ballot(keep);
@@ -326,6 +448,24 @@ __global__ void k_l_match(LDev d, int r, uint32_t seed)
             }
         }
     }
+    n_mat = block_sum(n_mat);
+    n_dead = block_sum(n_dead);
+    if (threadIdx.x == 0) {
+        d.part[2 * LPART + blockIdx.x] = n_mat;
+        d.part[3 * LPART + blockIdx.x] = n_dead;
+    }
+}
+
+// ctr[r][c] = sum of the round's per-block partial counts (one block).
+__global__ void k_l_reduce(const int32_t *part, int nsel, int nmatch, int32_t *ctr_r)
+{
+    const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nb = c < 2 ? nsel : nmatch;
+    int x = 0;
+    for (int b = lane; b < nb; b += 64) x += part[c * LPART + b];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    if (lane == 0) ctr_r[c] = x;
 }
 
 // ---------------------------------------------------------------------------
@@ -583,6 +723,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         (rc = graph_buf(g, &g->d_lsamp, 2, "trace counters")) != MM_OK)
         return rc;
     if (trace && (rc = graph_buf(g, &g->d_lfeat, nn * NF, "features")) != MM_OK) return rc;
+    if ((rc = graph_buf(g, &g->d_lpart, (size_t)LCTR * LPART, "partial counts")) != MM_OK) return rc;
     if (!g->h_lctr && hipHostMalloc((void **)&g->h_lctr, sizeof(int32_t) * LSLOTS * LCTR) != hipSuccess)
         return mm::set_error(MM_ERR_NOMEM, "pinned host allocation failed");
 
@@ -605,7 +746,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
     d.reqw = g->d_lreqw;
     d.feat = g->d_lfeat;
     d.ctr = g->d_lctr;
-    d.params = g->d_params;
+    d.part = g->d_lpart;
     d.scount = g->d_lsamp;
     if (trace) {
         const size_t cap = (size_t)lo->trace_cap;
@@ -619,8 +760,10 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         ML_TRY(hipMemcpyAsync(g->d_params, lo->params, sizeof(float) * NP, hipMemcpyHostToDevice, st),
                "Unable to set network parameters");
 
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)g->n + block - 1) / block,
-                                                                  (int64_t)g->num_cu * (2048 / block)));
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>({((int64_t)g->n + block - 1) / block,
+                                                                   (int64_t)g->num_cu * (2048 / block), LPART}));
+    const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>({((int64_t)g->n + LTILE - 1) / LTILE,
+                                                                    (int64_t)g->num_cu * 8, LPART}));
     ML_TRY(hipEventRecord(g->ev_t1, st), "hipEventRecord");
     hipLaunchKernelGGL(k_l_fill, dim3(1), dim3(256), 0, st, d.ctr, (int64_t)LSLOTS * LCTR, 0);
     hipLaunchKernelGGL(k_l_fill, dim3(1), dim3(64), 0, st, (int32_t *)d.scount, (int64_t)4, 0);
@@ -633,15 +776,19 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         for (int rr = r; rr <= end; ++rr) {
             const uint32_t seed = o->seeds[rr - 1];
             if (A8) {
-                hipLaunchKernelGGL(k_l_select<true>, dim3(grid), dim3(block), 0, st, d, rr, seed);
-                hipLaunchKernelGGL(k_l_request<true>, dim3(grid), dim3(block), 0, st, d);
-                hipLaunchKernelGGL(k_l_respond<true>, dim3(grid), dim3(block), 0, st, d);
+                hipLaunchKernelGGL((k_l_select<true, SEL_G>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed,
+                                   (const float *)g->d_params);
+                hipLaunchKernelGGL((k_l_request<true, SCAN_G>), dim3(grid), dim3(block), 0, st, d);
+                hipLaunchKernelGGL((k_l_respond<true, SCAN_G>), dim3(grid), dim3(block), 0, st, d);
             } else {
-                hipLaunchKernelGGL(k_l_select<false>, dim3(grid), dim3(block), 0, st, d, rr, seed);
-                hipLaunchKernelGGL(k_l_request<false>, dim3(grid), dim3(block), 0, st, d);
-                hipLaunchKernelGGL(k_l_respond<false>, dim3(grid), dim3(block), 0, st, d);
+                hipLaunchKernelGGL((k_l_select<false, SEL_G>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed,
+                                   (const float *)g->d_params);
+                hipLaunchKernelGGL((k_l_request<false, SCAN_G>), dim3(grid), dim3(block), 0, st, d);
+                hipLaunchKernelGGL((k_l_respond<false, SCAN_G>), dim3(grid), dim3(block), 0, st, d);
             }
             hipLaunchKernelGGL(k_l_match, dim3(grid), dim3(block), 0, st, d, rr, seed);
+            hipLaunchKernelGGL(k_l_reduce, dim3(1), dim3(64 * LCTR), 0, st, (const int32_t *)d.part, sgrid, grid,
+                               d.ctr + (size_t)rr * LCTR);
         }
         ML_TRY(hipGetLastError(), "A HIP error occurred during the matching process");
         if (maximal) {

@@ -113,11 +113,11 @@ def compare(csr, params, seeds=None, graph=None, max_rounds: int = 0) -> dict:
     learned = g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
     wp, sp = mmamd.get_weight(csr, plain.match)
     wl, sl = mmamd.get_weight(csr, learned.match)
-    reach = None
+    reach, reach_ms = None, None
     for r in range(1, learned.rounds + 1):
         m = g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=False, seeds=seeds, max_rounds=r)
         if mmamd.get_weight(csr, m.match)[0] >= wp:
-            reach = r
+            reach, reach_ms = r, m.match_ms
             break
     # the same kernels with the plain colouring: the cost of the learned rounds without the network
     through = g.match_learned(None, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
@@ -126,7 +126,7 @@ def compare(csr, params, seeds=None, graph=None, max_rounds: int = 0) -> dict:
                   "valid": mmamd.test_matching(csr, plain.match)[0]},
         "learned": {"rounds": learned.rounds, "matched": sl, "weight": wl, "ms": learned.match_ms,
                     "valid": mmamd.test_matching(csr, learned.match)[0],
-                    "rounds_to_plain_weight": reach},
+                    "rounds_to_plain_weight": reach, "ms_to_plain_weight": reach_ms},
         "plain_colouring_in_learned_kernels_ms": through.match_ms,
         "weight_ratio": wl / wp if wp else None,
     }
