@@ -1,23 +1,31 @@
 """Benchmark: GTEPS to >= 99% matched on a weighted random geometric graph.
 
-Workload: n = 2^22 points uniform in the unit square (seed 12345), edges between
-points closer than r = 0.55 sqrt(ln n / n), float32 weights uniform in (0, 1]
-mirrored per undirected edge; heaviest-red-neighbour matching with index
-tie-break (the reference's weighted GPU matcher, type 9), seeds from glibc
-rand() after srand(12345).
+Default workload (rgg22): n = 2^22 points uniform in the unit square (seed
+12345), edges between points closer than r = 0.55 sqrt(ln n / n), float32
+weights (k + 1) / 2^24 mirrored per undirected edge; heaviest-red-neighbour
+matching with index tie-break (the reference's weighted GPU matcher, type 9),
+seeds from glibc rand() after srand(12345).
 
 A step is one matching of the graph, already resident in HBM, up to t_99: the
 end of the first round whose matched-vertex count reaches 99% of the count at
 convergence (the run is deterministic, so the round index is fixed).  value =
-m / t_99 / 1e9 with m = undirected edges.  The JSON line also carries the
-roofline of the dominant kernel (HIP events on the library's stream) and the
-CPU baseline (the OpenMP restatement of the reference's TBB round matcher,
-type 11, timed on this host), and the learned selection against the plain
-rounds on the same graph: rounds, matched vertices, weight and device time to
+m / t_99 / 1e9 with m = undirected edges.  The tile configuration (lanes per
+row, block size, hub degree) is the autotuner's pick for this step, checked to
+give the same match array as the defaults.  The JSON line also carries the
+roofline of the dominant kernel (HIP events on the library's stream), the CPU
+baseline (the OpenMP restatement of the reference's TBB round matcher, type
+11, timed on this host), and the learned selection against the plain rounds on
+the same graph: rounds, matched vertices, weight and device time to
 convergence, and the rounds and time the learned run needs to reach the plain
 run's final weight.
 
-Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--log2n 22]
+Other workloads (one JSON line each, not the headline):
+  --workload rmat24    R-MAT scale 24, edge factor 16, the same step and metric
+  --workload batch4096 4096 graphs (n log-uniform in [2^8, 2^16], half grids,
+                       half RGGs) matched to convergence by one launch
+                       sequence per round chunk; GTEPS over all their edges
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload rgg22]
 """
 from __future__ import annotations
 
@@ -33,6 +41,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "multicore-matching_amd"))
 
 import mmamd  # noqa: E402
+import workloads  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rgg22_weighted_t99_kernels.md")
@@ -68,16 +77,18 @@ def kernel_bytes(res: mmamd.MatchResult, rounds: int, push: bool) -> dict:
     return out
 
 
-def measured_traffic(workload: str, kernel: str):
+def measured_traffic(workload: str, kernel: str, tiles: dict):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 byte-counter
     summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy calibration),
-    or None when that summary covers another workload or kernel."""
+    or None when that summary covers another workload, kernel or tile choice."""
     try:
         text = open(TRAFFIC_FILE).read()
         rec = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
     except (OSError, IndexError, ValueError):
         return None
     if rec.get("workload") != workload or rec.get("kernel") != kernel:
+        return None
+    if rec.get("tiles") is not None and rec["tiles"] != tiles:
         return None
     return rec.get("traffic_bytes_per_launch")
 
@@ -96,8 +107,177 @@ def cpu_baseline(csr, seeds, r99: int, t_budget_s: float = 20.0) -> dict:
         oracle.match(csr, True, False, seeds, max_rounds=r99, nthreads=threads)
     t = (time.perf_counter() - t0) / reps
     return {"value": csr.n_edges / t / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"same RGG, first {r99} rounds (to 99% matched), OpenMP restatement of the reference's "
+            "sample": f"same graph, first {r99} rounds (to 99% matched), OpenMP restatement of the reference's "
                       f"TBB weighted round matcher (type 11), {reps} timed runs, {t * 1e3:.1f} ms each"}
+
+
+def tiles_of(t: dict) -> dict:
+    return {"lanes": t["lanes_per_vertex"], "block_size": t["block_size"], "hub_degree": t["hub_degree"],
+            "unroll": t["entries_per_thread"]}
+
+
+def timed(fn, warmup: int, steps: int) -> float:
+    for _ in range(warmup):
+        fn()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    return (time.perf_counter() - t0) / steps
+
+
+def run_single(args, csr, name: str, workload_desc: str) -> dict:
+    n = csr.n
+    t0 = time.perf_counter()
+    g = mmamd.Graph(csr, keep_order=args.keep_order)
+    log(f"graph upload (incl. renumbering) {time.perf_counter() - t0:.2f} s")
+    seeds = mmamd.reference_seeds(mmamd.MM_MAX_ROUNDS)
+    base = dict(respond=args.respond)
+
+    full = g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, **base)
+    final = int(full.matched[-1])
+    r99 = int(np.argmax(full.matched >= 0.99 * final)) + 1
+    weight, size = mmamd.get_weight(csr, full.match)
+    valid = mmamd.test_matching(csr, full.match)
+    log(f"maximal run: rounds={full.rounds} matched={final} ({final / n:.4f}) weight={weight:.1f} "
+        f"r99={r99} valid={valid} respond={full.respond} match_ms={full.match_ms:.3f}")
+
+    if args.tune:
+        tune = g.autotune(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, **base)
+        if tune["mismatches"]:
+            raise RuntimeError(f"autotuner candidates disagree: {tune}")
+        tiles = tiles_of(tune)
+    else:
+        tune = None
+        tiles = {"lanes": args.lanes, "block_size": args.block, "hub_degree": args.hub, "unroll": 0}
+    log(f"tiles {tiles} (autotune {tune})")
+
+    def step():
+        g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, fetch=False, stats=False,
+                **tiles, **base)
+
+    wall = timed(step, args.warmup, args.steps)
+    ev = [g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, fetch=False, **tiles,
+                  **base).match_ms for _ in range(5)]
+    t99_event_ms = float(np.median(ev))
+    log(f"t99: wall {wall * 1e3:.4f} ms/step, device events {t99_event_ms:.4f} ms")
+    # the tuned step's result is the default step's result
+    a = g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, **tiles, **base)
+    b = g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, **base)
+    if not np.array_equal(a.match, b.match):
+        raise RuntimeError("tuned tiles changed the match array")
+
+    prof = g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, fetch=False, profile=True,
+                   **tiles, **base)
+    push = prof.respond == mmamd.MM_RESPOND_PUSH
+    kb = kernel_bytes(prof, r99, push)
+    kms = {k: v for k, v in prof.kernel_ms.items() if k != "counters"}
+    dom = max(kms, key=kms.get)
+    launches = prof.kernel_launches[dom]
+    achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
+    log(f"profile: kernel_ms={kms} launches={prof.kernel_launches} bytes={kb}")
+    traffic = None
+    if not args.keep_order and args.respond == 0:
+        traffic = measured_traffic(f"{name}_weighted_t99", dom, tiles)
+
+    line = {
+        "metric": f"GTEPS to >=99% matched ({workload_desc}, weighted)",
+        "value": round(csr.n_edges / wall / 1e9, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "int32+f32cmp",
+        "data": "synthetic (seeded, generated on the host)",
+        "config": {"workload": f"{name}: {workload_desc}" + (", r=0.55*sqrt(ln n/n)" if name.startswith("rgg") else "")
+                               + f", seed {args.seed}, type 9 rounds to t_99", "n": n,
+                   "m": csr.n_edges, "nnz": csr.nnz, "rounds_to_t99": r99, "rounds_to_convergence": full.rounds,
+                   "respond": "push" if push else "pull", "tiles": tiles, "parallelism": "single GPU"},
+        "t99_device_event_ms": round(t99_event_ms, 5),
+        "gteps_device_events": round(csr.n_edges / (t99_event_ms * 1e-3) / 1e9, 3),
+        "matched_fraction": final / n,
+        "matched_vertices": final,
+        "matching_weight": weight,
+        "valid": bool(valid[0]),
+        "autotune": tune,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "launches": launches, "avg_launch_us": round(kms[dom] * 1e3 / max(1, launches), 3),
+                     "bytes_per_launch": kb[dom] / max(1, launches)},
+        "round_roofline_frac": round(kb["round_total"] / (t99_event_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
+    }
+    if name == "rgg22" and not args.no_learned:
+        import learn
+
+        cmp = learn.compare(csr, learn.load_params(args.params), seeds=seeds, graph=g)
+        cmp["params"] = os.path.relpath(args.params, ROOT)
+        line["learned_vs_plain"] = cmp
+        log(f"learned vs plain: {json.dumps(cmp)}")
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        line["cpu_baseline"] = cpu_baseline(csr, seeds, r99)
+    g.close()
+    return line
+
+
+def run_batch(args) -> dict:
+    t0 = time.perf_counter()
+    specs = workloads.batch_sizes(args.batch_graphs, seed=args.seed)
+    graphs = [workloads.batch_graph(*s) for s in specs]
+    n_tot = sum(g.n for g in graphs)
+    m_tot = sum(g.n_edges for g in graphs)
+    log(f"generated {len(graphs)} graphs, {n_tot} vertices, {m_tot} edges in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    b = mmamd.Batch(graphs)
+    log(f"batch upload {time.perf_counter() - t0:.1f} s")
+    seeds = mmamd.reference_seeds(mmamd.MM_MAX_ROUNDS)
+    res, per_graph, rounds = b.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds)
+    log(f"batch maximal: rounds={res.rounds} (per graph {rounds.min()}..{rounds.max()}) match_ms={res.match_ms:.3f}")
+    # every graph's result is the single-graph path's (a seeded sample of graphs, with the smallest and largest)
+    rng = np.random.default_rng(args.seed)
+    sizes = np.array([g.n for g in graphs])
+    pick = sorted(set(rng.choice(len(graphs), size=min(args.batch_check, len(graphs)), replace=False).tolist()
+                      + [int(np.argmin(sizes)), int(np.argmax(sizes))]))
+    bad = 0
+    for k in pick:
+        single = mmamd.Graph(graphs[k]).match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds)
+        bad += int(not np.array_equal(single.match, per_graph[k]) or single.rounds != rounds[k])
+    union, vo = workloads.concat(graphs)
+    valid = mmamd.test_matching(union, workloads.batch_match_global(res.match, vo))
+    log(f"checked {len(pick)} graphs against the single-graph path: {bad} differ; union valid={valid}")
+    tune = b.autotune(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds) if args.tune else None
+    tiles = tiles_of(tune) if tune else {"lanes": 0, "block_size": 0, "hub_degree": 0, "unroll": 0}
+    log(f"tiles {tiles} (autotune {tune})")
+
+    def step():
+        b.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, fetch=False, **tiles)
+
+    wall = timed(step, args.warmup, args.steps)
+    ev = float(np.median([b.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, fetch=False,
+                                  **tiles)[0].match_ms for _ in range(5)]))
+    tuned, per_tuned, _ = b.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, **tiles)
+    if not np.array_equal(tuned.match, res.match):
+        raise RuntimeError("tuned tiles changed the batch result")
+    weight, size = mmamd.get_weight(union, workloads.batch_match_global(res.match, vo))
+    return {
+        "metric": "GTEPS, 4096-graph batch matched to convergence (weighted)",
+        "value": round(m_tot / wall / 1e9, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3, 4), "higher_is_better": True, "scaling": "strong",
+        "vs_baseline": None, "dtype": "int32+f32cmp", "data": "synthetic (seeded, generated on the host)",
+        "config": {"workload": f"batch{len(graphs)}: n log-uniform in [2^8, 2^16], even graphs 2-D grids "
+                               f"(4-neighbour), odd graphs RGG r=0.55*sqrt(ln n/n), seed {args.seed}; type 9 "
+                               f"rounds to convergence", "graphs": len(graphs), "n": n_tot, "m": m_tot,
+                   "rounds": res.rounds, "rounds_per_graph": [int(rounds.min()), int(np.median(rounds)),
+                                                              int(rounds.max())],
+                   "tiles": tiles, "parallelism": "single GPU"},
+        "device_event_ms": round(ev, 4), "gteps_device_events": round(m_tot / (ev * 1e-3) / 1e9, 3),
+        "matched_vertices": size, "matching_weight": weight, "valid": bool(valid[0]),
+        "graphs_checked_against_single_path": len(pick), "graphs_differing": bad, "autotune": tune,
+    }
 
 
 def main():
@@ -105,10 +285,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--log2n", type=int, default=22)
+    ap.add_argument("--workload", choices=("rgg22", "rmat24", "batch4096"), default="rgg22")
+    ap.add_argument("--log2n", type=int, default=22, help="rgg workload size")
+    ap.add_argument("--scale", type=int, default=24, help="rmat workload scale")
+    ap.add_argument("--batch-graphs", type=int, default=4096)
+    ap.add_argument("--batch-check", type=int, default=32, help="batch graphs checked against the single path")
     ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--no-tune", dest="tune", action="store_false", help="fixed tiles instead of the autotuner")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--hub", type=int, default=0)
     ap.add_argument("--respond", type=int, default=mmamd.MM_RESPOND_AUTO)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-selection comparison")
@@ -136,91 +322,18 @@ def main():
         print("bench.py: no GPU visible to the HIP runtime", file=sys.stderr)
         sys.exit(1)
 
-    n = 1 << args.log2n
     t0 = time.perf_counter()
-    csr = mmamd.gen_rgg(n, seed=args.seed, weighted=True)
-    log(f"generated RGG n={n} nnz={csr.nnz} m={csr.n_edges} in {time.perf_counter() - t0:.2f} s")
-    t0 = time.perf_counter()
-    g = mmamd.Graph(csr, keep_order=args.keep_order)
-    log(f"graph upload (incl. renumbering) {time.perf_counter() - t0:.2f} s")
-    seeds = mmamd.reference_seeds(mmamd.MM_MAX_ROUNDS)
-    tiles = dict(lanes=args.lanes, block_size=args.block, respond=args.respond)
-
-    full = g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, **tiles)
-    final = int(full.matched[-1])
-    r99 = int(np.argmax(full.matched >= 0.99 * final)) + 1
-    weight, size = mmamd.get_weight(csr, full.match)
-    valid = mmamd.test_matching(csr, full.match)
-    log(f"maximal run: rounds={full.rounds} matched={final} ({final / n:.4f}) weight={weight:.1f} "
-        f"r99={r99} valid={valid} respond={full.respond} lanes={full.lanes} match_ms={full.match_ms:.3f}")
-
-    def step():
-        g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, fetch=False, stats=False,
-                **tiles)
-
-    for _ in range(args.warmup):
-        step()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    wall = (time.perf_counter() - t0) / args.steps
-    ev = [g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, fetch=False, **tiles).match_ms
-          for _ in range(5)]
-    t99_event_ms = float(np.median(ev))
-    log(f"t99: wall {wall * 1e3:.4f} ms/step, device events {t99_event_ms:.4f} ms")
-
-    prof = g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, fetch=False, profile=True,
-                   **tiles)
-    push = prof.respond == mmamd.MM_RESPOND_PUSH
-    kb = kernel_bytes(prof, r99, push)
-    kms = {k: v for k, v in prof.kernel_ms.items() if k != "counters"}
-    dom = max(kms, key=kms.get)
-    launches = prof.kernel_launches[dom]
-    achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
-    log(f"profile: kernel_ms={kms} launches={prof.kernel_launches} bytes={kb}")
-
-    default_tiles = not args.keep_order and args.lanes == 0 and args.block == 0 and args.respond == 0
-    traffic = measured_traffic("rgg22_weighted_t99", dom) if args.log2n == 22 and default_tiles else None
-    line = {
-        "metric": "GTEPS to >=99% matched (2^22 RGG, weighted)",
-        "value": round(csr.n_edges / wall / 1e9, 3),
-        "unit": "GTEPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3, 5),
-        "higher_is_better": True,
-        "scaling": "strong",
-        "vs_baseline": None,
-        "dtype": "int32+f32cmp",
-        "data": "synthetic (seeded RGG, generated on the host)",
-        "config": {"workload": f"weighted RGG n=2^{args.log2n}, r=0.55*sqrt(ln n/n), seed {args.seed}, "
-                               f"type 9 rounds to t_99", "n": n, "m": csr.n_edges, "nnz": csr.nnz,
-                   "rounds_to_t99": r99, "rounds_to_convergence": full.rounds, "respond": "push" if push else "pull",
-                   "lanes_per_vertex": full.lanes, "parallelism": "single GPU"},
-        "t99_device_event_ms": round(t99_event_ms, 5),
-        "gteps_device_events": round(csr.n_edges / (t99_event_ms * 1e-3) / 1e9, 3),
-        "matched_fraction": final / n,
-        "matched_vertices": final,
-        "matching_weight": weight,
-        "valid": bool(valid[0]),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "launches": launches, "avg_launch_us": round(kms[dom] * 1e3 / max(1, launches), 3),
-                     "bytes_per_launch": kb[dom] / max(1, launches)},
-        "round_roofline_frac": round(kb["round_total"] / (t99_event_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
-    }
-    if not args.no_learned:
-        import learn
-
-        cmp = learn.compare(csr, learn.load_params(args.params), seeds=seeds, graph=g)
-        cmp["params"] = os.path.relpath(args.params, ROOT)
-        line["learned_vs_plain"] = cmp
-        log(f"learned vs plain: {json.dumps(cmp)}")
-    if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        line["cpu_baseline"] = cpu_baseline(csr, seeds, r99)
+    if args.workload == "batch4096":
+        line = run_batch(args)
+    elif args.workload == "rmat24":
+        csr = workloads.rmat(args.scale, seed=args.seed)
+        log(f"generated R-MAT scale {args.scale}: n={csr.n} nnz={csr.nnz} in {time.perf_counter() - t0:.2f} s")
+        line = run_single(args, csr, f"rmat{args.scale}", f"R-MAT scale {args.scale}, edge factor 16, "
+                                                          f"(0.57, 0.19, 0.19, 0.05)")
+    else:
+        csr = workloads.rgg(args.log2n, seed=args.seed)
+        log(f"generated RGG n={csr.n} nnz={csr.nnz} m={csr.n_edges} in {time.perf_counter() - t0:.2f} s")
+        line = run_single(args, csr, f"rgg{args.log2n}", f"2^{args.log2n} RGG")
     print(json.dumps(line), flush=True)
 
 

@@ -149,6 +149,30 @@ int32_t mm_batch_destroy(mm_batch *b);
 int32_t mm_batch_match(mm_batch *b, const mm_opts *opts, int32_t *match_out,
                        int32_t *rounds_out, mm_stats *stats);
 
+/* Autotuner (no reference counterpart; the reference fixes 256 threads per
+ * block, bin/match.cpp:186).  Searches, one parameter at a time, the lanes per
+ * row (a row is scanned by 1..64 lanes, or by entry-parallel tiles with 4, 8
+ * or 16 entries per thread), the block size (64..1024) and the hub degree
+ * (rows longer than it are scanned by a whole block), timing each candidate
+ * with device events on this graph and these seeds, and returns the fastest.
+ * Tiles never change results: with verify = 1 every candidate's match array
+ * is compared with the first one's and differences are counted. */
+typedef struct mm_tune_opts {
+    int32_t reps;     /* timed runs per candidate, the median is kept; 0 = 3 */
+    int32_t verify;   /* 1: compare every candidate's match array with the first */
+} mm_tune_opts;
+typedef struct mm_tune_result {
+    int32_t block_size, lanes_per_vertex, hub_degree, entries_per_thread;  /* the fastest candidate */
+    float best_ms;        /* its median match_ms */
+    float default_ms;     /* median match_ms of the first candidate (the opts as given) */
+    int32_t candidates;   /* candidates timed */
+    int32_t mismatches;   /* candidates whose match array differed (verify = 1) */
+} mm_tune_result;
+/* opts: the run to tune (kind, maximal, max_rounds, seeds, barrier, respond);
+ * its tile fields are the search's starting point (0 = defaults). */
+int32_t mm_autotune(mm_graph *g, const mm_opts *opts, const mm_tune_opts *t, mm_tune_result *out);
+int32_t mm_batch_autotune(mm_batch *b, const mm_opts *opts, const mm_tune_opts *t, mm_tune_result *out);
+
 /* Result metrics of a match array (host; src/matchcpu.cu:64-97 getWeight and
  * 127-246 testMatching).  mm_get_weight: size = matched VERTICES, weight summed
  * in double over both endpoints and halved.  mm_test_matching fills

@@ -1542,6 +1542,109 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
 
 }  // namespace
 
+namespace {
+
+// Coordinate search over lanes per row, block size and hub degree (see
+// mm_autotune in mm_match.h).  Every candidate runs once untimed (it captures
+// its launch sequence), then reps times; the median match_ms counts.
+int32_t autotune(mm_graph *g, const mm_opts *base, const mm_tune_opts *t, mm_tune_result *out)
+{
+    if (!g || !base || !out) return mm::set_error(MM_ERR_INVALID, "mm_autotune: NULL argument");
+    const int reps = t && t->reps > 0 ? t->reps : 3;
+    const bool verify = t && t->verify;
+    std::memset(out, 0, sizeof(*out));
+    std::vector<int32_t> ref, cur;
+    if (verify) {
+        ref.resize((size_t)g->n);
+        cur.resize((size_t)g->n);
+    }
+    struct Cand {
+        int32_t block, lanes, hub, unroll;
+    };
+    Cand best{base->block_size, base->lanes_per_vertex, base->hub_degree, base->entries_per_thread};
+    float best_ms = 0.0f;
+    bool have = false;
+    auto time_cand = [&](const Cand &c, float *ms_out) -> int32_t {
+        mm_opts o = *base;
+        o.block_size = c.block;
+        o.lanes_per_vertex = c.lanes;
+        o.hub_degree = c.hub;
+        o.entries_per_thread = c.unroll;
+        o.profile = 0;
+        mm_stats st;
+        std::vector<float> ms;
+        for (int k = 0; k <= reps; ++k) {
+            const int32_t rc = run_matching(g, &o, &st, nullptr);
+            if (rc != MM_OK) return rc;
+            if (k > 0) ms.push_back(st.match_ms);
+        }
+        std::sort(ms.begin(), ms.end());
+        *ms_out = ms[ms.size() / 2];
+        ++out->candidates;
+        if (verify && g->n > 0) {
+            const int32_t rc = mm_graph_copy_match(g, have ? cur.data() : ref.data());
+            if (rc != MM_OK) return rc;
+            if (have && cur != ref) ++out->mismatches;
+        }
+        return MM_OK;
+    };
+    auto consider = [&](const Cand &c) -> int32_t {
+        float ms = 0.0f;
+        const int32_t rc = time_cand(c, &ms);
+        if (rc != MM_OK) return rc;
+        if (!have) out->default_ms = ms;
+        if (!have || ms < best_ms) {
+            best = c;
+            best_ms = ms;
+        }
+        have = true;
+        return MM_OK;
+    };
+    int32_t rc = consider(best);
+    if (rc != MM_OK) return rc;
+    // lanes per row (entry-parallel tiles with 4, 8, 16 entries per thread)
+    {
+        const Cand b0 = best;
+        for (int lanes : {1, 2, 4, 8, 16, 32, 64}) {
+            Cand c = b0;
+            c.lanes = lanes;
+            c.unroll = 0;
+            if ((rc = consider(c)) != MM_OK) return rc;
+        }
+        for (int u : {4, 8, 16}) {
+            Cand c = b0;
+            c.lanes = MM_LANES_TILES;
+            c.unroll = u;
+            if ((rc = consider(c)) != MM_OK) return rc;
+        }
+    }
+    {
+        const Cand b0 = best;
+        for (int block : {64, 128, 256, 512, 1024}) {
+            Cand c = b0;
+            c.block = block;
+            if ((rc = consider(c)) != MM_OK) return rc;
+        }
+    }
+    if (g->max_degree > 32) {
+        const Cand b0 = best;
+        for (int hub : {32, 64, 128, 256, 512, 1024, 4096}) {
+            if (hub >= g->max_degree && hub != 256) continue;
+            Cand c = b0;
+            c.hub = hub;
+            if ((rc = consider(c)) != MM_OK) return rc;
+        }
+    }
+    out->block_size = best.block > 0 ? best.block : 256;
+    out->lanes_per_vertex = best.lanes != 0 ? best.lanes : 4;
+    out->hub_degree = best.hub > 0 ? best.hub : 256;
+    out->entries_per_thread = best.lanes == MM_LANES_TILES ? (best.unroll > 0 ? best.unroll : 4) : 0;
+    out->best_ms = best_ms;
+    return MM_OK;
+}
+
+}  // namespace
+
 int32_t mm_graph_alloc(mm_graph *g, void **p, size_t bytes, const char *what)
 {
     char *q = nullptr;
@@ -1691,6 +1794,19 @@ int32_t mm_batch_destroy(mm_batch *b)
     release(b->g);
     delete b;
     return MM_OK;
+}
+
+int32_t mm_autotune(mm_graph *g, const mm_opts *opts, const mm_tune_opts *t, mm_tune_result *out)
+{
+    mm::clear_error();
+    return autotune(g, opts, t, out);
+}
+
+int32_t mm_batch_autotune(mm_batch *b, const mm_opts *opts, const mm_tune_opts *t, mm_tune_result *out)
+{
+    mm::clear_error();
+    if (!b || !b->g) return mm::set_error(MM_ERR_INVALID, "mm_batch_autotune: batch is NULL");
+    return autotune(b->g, opts, t, out);
 }
 
 int32_t mm_batch_match(mm_batch *b, const mm_opts *opts, int32_t *match_out, int32_t *rounds_out, mm_stats *stats)

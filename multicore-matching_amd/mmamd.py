@@ -139,6 +139,27 @@ class _TrainOpts(ctypes.Structure):
     ]
 
 
+class _TuneOpts(ctypes.Structure):
+    _fields_ = [("reps", ctypes.c_int32), ("verify", ctypes.c_int32)]
+
+
+class _TuneResult(ctypes.Structure):
+    _fields_ = [
+        ("block_size", ctypes.c_int32),
+        ("lanes_per_vertex", ctypes.c_int32),
+        ("hub_degree", ctypes.c_int32),
+        ("entries_per_thread", ctypes.c_int32),
+        ("best_ms", ctypes.c_float),
+        ("default_ms", ctypes.c_float),
+        ("candidates", ctypes.c_int32),
+        ("mismatches", ctypes.c_int32),
+    ]
+
+
+def _tune_dict(r: _TuneResult) -> dict:
+    return {k: getattr(r, k) for k, _ in _TuneResult._fields_}
+
+
 # Every symbol include/*.h declares, with its ctypes signature.
 _P = ctypes.c_void_p
 _I32, _I64, _U64, _D = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
@@ -160,6 +181,9 @@ SYMBOLS = {
     "mm_csr_free": (None, [ctypes.POINTER(_Csr)]),
     "mm_get_weight": (_I32, [_I32, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
     "mm_test_matching": (_I32, [_I32, _P, _P, _P, _P]),
+    "mm_autotune": (_I32, [_P, ctypes.POINTER(_Opts), ctypes.POINTER(_TuneOpts), ctypes.POINTER(_TuneResult)]),
+    "mm_batch_autotune": (_I32, [_P, ctypes.POINTER(_Opts), ctypes.POINTER(_TuneOpts),
+                                 ctypes.POINTER(_TuneResult)]),
     "mm_match_learned": (_I32, [_P, ctypes.POINTER(_Opts), ctypes.POINTER(_LearnOpts), _P, ctypes.POINTER(_Stats)]),
     "mm_mlp_init": (_I32, [_U64, _P]),
     "mm_mlp_forward": (_I32, [_I32, _P, _P, _I64, _P]),
@@ -420,6 +444,19 @@ class Graph:
         samples["offered"] = int(lo.trace_offered)
         return res, samples
 
+    def autotune(self, kind: int = MM_WEIGHTED, maximal: bool = True, seeds=None, max_rounds: int = 0,
+                 barrier: int = MM_DEFAULT_BARRIER, respond: int = MM_RESPOND_AUTO, reps: int = 3,
+                 verify: bool = True) -> dict:
+        """Fastest block size / lanes per row / hub degree for this graph and run (mm_autotune)."""
+        if seeds is None:
+            seeds = reference_seeds(MM_MAX_ROUNDS)
+        o, keep = _make_opts(kind, maximal, seeds, max_rounds, barrier, 0, 0, 0, respond, False)
+        r = _TuneResult()
+        _check(lib().mm_autotune(self._h, ctypes.byref(o), ctypes.byref(_TuneOpts(reps, int(verify))),
+                                 ctypes.byref(r)))
+        del keep
+        return _tune_dict(r)
+
     def match_type(self, type_code: int, seeds=None, **kw) -> MatchResult:
         kind, maximal = TYPE_TO_KIND[type_code]
         return self.match(kind=kind, maximal=bool(maximal), seeds=seeds, **kw)
@@ -482,6 +519,18 @@ class Batch:
         if out is not None:
             per_graph = [out[self.vertex_offsets[k]:self.vertex_offsets[k + 1]] for k in range(len(self.graphs))]
         return res, per_graph, rounds
+
+    def autotune(self, kind: int = MM_WEIGHTED, maximal: bool = True, seeds=None, max_rounds: int = 0,
+                 barrier: int = MM_DEFAULT_BARRIER, respond: int = MM_RESPOND_AUTO, reps: int = 3,
+                 verify: bool = True) -> dict:
+        if seeds is None:
+            seeds = reference_seeds(MM_MAX_ROUNDS)
+        o, keep = _make_opts(kind, maximal, seeds, max_rounds, barrier, 0, 0, 0, respond, False)
+        r = _TuneResult()
+        _check(lib().mm_batch_autotune(self._h, ctypes.byref(o), ctypes.byref(_TuneOpts(reps, int(verify))),
+                                       ctypes.byref(r)))
+        del keep
+        return _tune_dict(r)
 
     def close(self):
         if getattr(self, "_h", None):
