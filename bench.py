@@ -156,6 +156,12 @@ def run_single(args, csr, name: str, workload_desc: str) -> dict:
                 **tiles, **base)
 
     wall = timed(step, args.warmup, args.steps)
+
+    def step_to_host():
+        g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, fetch=True, stats=False,
+                **tiles, **base)
+
+    wall_host = timed(step_to_host, 1, max(3, args.steps // 4))
     ev = [g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, fetch=False, **tiles,
                   **base).match_ms for _ in range(5)]
     t99_event_ms = float(np.median(ev))
@@ -198,6 +204,7 @@ def run_single(args, csr, name: str, workload_desc: str) -> dict:
                    "respond": "push" if push else "pull", "tiles": tiles, "parallelism": "single GPU"},
         "t99_device_event_ms": round(t99_event_ms, 5),
         "gteps_device_events": round(csr.n_edges / (t99_event_ms * 1e-3) / 1e9, 3),
+        "gteps_with_result_copy_to_host": round(csr.n_edges / wall_host / 1e9, 3),
         "matched_fraction": final / n,
         "matched_vertices": final,
         "matching_weight": weight,
