@@ -83,14 +83,13 @@ def measured_traffic(workload: str, kernel: str, tiles: dict):
     or None when that summary covers another workload, kernel or tile choice."""
     try:
         text = open(TRAFFIC_FILE).read()
-        rec = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
+        recs = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
     except (OSError, IndexError, ValueError):
         return None
-    if rec.get("workload") != workload or rec.get("kernel") != kernel:
-        return None
-    if rec.get("tiles") is not None and rec["tiles"] != tiles:
-        return None
-    return rec.get("traffic_bytes_per_launch")
+    for rec in recs if isinstance(recs, list) else [recs]:
+        if rec.get("workload") == workload and rec.get("kernel") == kernel and rec.get("tiles") == tiles:
+            return rec.get("traffic_bytes_per_launch")
+    return None
 
 
 def cpu_baseline(csr, seeds, r99: int, t_budget_s: float = 20.0) -> dict:

@@ -74,6 +74,8 @@ struct Dev {
     unsigned long long *ctr64; // [ROUND_SLOTS][CTR64_STRIDE] degree sums (profile)
     unsigned long long *ctrb64;// [ROUND_SLOTS][2][nb] per-bucket degree sums (profile)
     unsigned long long *hub64; // [ROUND_SLOTS][2] degree sums of hub rows (profile)
+    int32_t *ctr_next;         // the other counter set, cleared here for the next run
+    unsigned long long *hub64_next;
     const uint32_t *seeds;     // seeds[r - 1] drives the select of round r
     const int32_t *loc;        // graph-local ORIGINAL id of each (internally renumbered) vertex:
                                // the hash input, the proposal tie-break id and the match value base
@@ -306,11 +308,6 @@ __device__ void first_scan_hubs(const Dev &d, const int4 *list, int32_t count, i
 }
 
 
-__global__ void k_clear(int32_t *ctr, unsigned long long *hub64, int n32, int n64)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n32; i += gridDim.x * blockDim.x) ctr[i] = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n64; i += gridDim.x * blockDim.x) hub64[i] = 0ull;
-}
 
 __device__ __forceinline__ int32_t *ctrb_ptr(const Dev &d, int r, int j) { return d.ctrb + ((size_t)r * 4 + j) * d.nb; }
 __device__ __forceinline__ unsigned long long *ctrb64_ptr(const Dev &d, int r, int j)
@@ -364,6 +361,13 @@ __global__ void k_init_select(Dev d)
     __shared__ int32_t s_cnt[2];
     const int parity = 1, b = bucket_of_block(d);
     if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
+    // Runs alternate between two counter sets; the hub-list sizes and hub
+    // degree sums are accumulated with atomics, so this run clears the other
+    // set for the next one (this run's set was cleared by the previous run).
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < ROUND_SLOTS * CTR_STRIDE; i += blockDim.x) d.ctr_next[i] = 0;
+        for (int i = threadIdx.x; i < ROUND_SLOTS * 2; i += blockDim.x) d.hub64_next[i] = 0ull;
+    }
     __syncthreads();
     const uint32_t seed = d.seeds[0];
     const int64_t lo = (int64_t)b * d.bcap, hi = min((int64_t)d.n, lo + d.bcap);
@@ -865,10 +869,26 @@ __global__ void k_reduce_counts(Dev d, int r0, int with_nnz)
 
 // Writes the state array back in the caller's vertex numbering: coalesced
 // writes, reads gathered from the (L2/MALL-resident) state array.
-__global__ void k_unpermute(const int32_t *match, const int32_t *new_of, int32_t *out, int32_t n)
+// Offsets of the eight groups of (output position, internal id) pairs.
+struct UnpermOff {
+    int32_t off[9];
+};
+
+// out[o] = match[v] for every pair {o, v}.  The pairs are grouped by the
+// eighth of the internal ids that v falls in (ordered by o within a group), and
+// workgroup i works on group i % 8: workgroups are dealt to the 8 XCDs round
+// robin, so each XCD gathers from one 1/8 slice of match[], which stays in its
+// own L2, instead of every XCD missing on random lines of the whole array.
+__global__ void k_unpermute(const int2 *__restrict__ pairs, UnpermOff g, const int32_t *__restrict__ match,
+                            int32_t *__restrict__ out)
 {
-    for (int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n; o += (int64_t)gridDim.x * blockDim.x)
-        out[o] = match[new_of[o]];
+    const int x = blockIdx.x & 7;
+    const int64_t step = (int64_t)(gridDim.x >> 3) * blockDim.x;
+    for (int64_t k = g.off[x] + (int64_t)(blockIdx.x >> 3) * blockDim.x + threadIdx.x; k < g.off[x + 1]; k += step) {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        const v2i p = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(pairs + k));
+        out[p.x] = match[p.y];
+    }
 }
 
 // Rounds each batched graph would execute alone: one plus the last round in
@@ -933,7 +953,7 @@ void release(mm_graph *g)
     void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_match, g->d_req,   g->d_st,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
-                    g->d_new_of, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_params,
+                    g->d_unperm, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_params,
                     g->d_lsamp};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -1076,13 +1096,16 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     g->list_cap = (int64_t)n + g->nb_max;
     if ((rc = dev_alloc(g, &g->d_lists, 2 * NLISTS * (size_t)g->list_cap, "work lists")) != MM_OK) return fail(rc);
     static_assert(sizeof(*g->d_lists) == 16, "work-list entries are 16 bytes");
-    if ((rc = dev_alloc(g, &g->d_ctr, (size_t)ROUND_SLOTS * CTR_STRIDE, "counters")) != MM_OK) return fail(rc);
+    if ((rc = dev_alloc(g, &g->d_ctr, (size_t)2 * ROUND_SLOTS * CTR_STRIDE, "counters")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_ctrb, (size_t)ROUND_SLOTS * 4 * g->nb_max, "bucket counters")) != MM_OK)
         return fail(rc);
     if ((rc = dev_alloc(g, &g->d_ctr64, (size_t)ROUND_SLOTS * CTR64_STRIDE, "counters")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_ctrb64, (size_t)ROUND_SLOTS * 2 * g->nb_max, "bucket counters")) != MM_OK)
         return fail(rc);
-    if ((rc = dev_alloc(g, &g->d_hub64, (size_t)ROUND_SLOTS * 2, "counters")) != MM_OK) return fail(rc);
+    if ((rc = dev_alloc(g, &g->d_hub64, (size_t)2 * ROUND_SLOTS * 2, "counters")) != MM_OK) return fail(rc);
+    if (hipMemset(g->d_ctr, 0, sizeof(int32_t) * 2 * ROUND_SLOTS * CTR_STRIDE) != hipSuccess ||
+        hipMemset(g->d_hub64, 0, sizeof(unsigned long long) * 2 * ROUND_SLOTS * 2) != hipSuccess)
+        return fail(mm::set_error(MM_ERR_HIP, "Unable to clear counters"));
     if ((rc = dev_alloc(g, &g->d_seeds, (size_t)MM_MAX_ROUNDS + 1, "seeds")) != MM_OK) return fail(rc);
     if (hipHostMalloc((void **)&g->h_ctr, sizeof(int32_t) * ROUND_SLOTS * CTR_STRIDE) != hipSuccess ||
         hipHostMalloc((void **)&g->h_ctr64, sizeof(unsigned long long) * ROUND_SLOTS * CTR64_STRIDE) != hipSuccess)
@@ -1157,9 +1180,23 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
             return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
     }
     if (g->relabelled) {
-        if ((rc = dev_alloc(g, &g->d_new_of, nn, "vertex order")) != MM_OK) return fail(rc);
+        // (output position, internal id) pairs grouped by the eighth of the
+        // internal ids, ordered by output position inside a group.
+        std::vector<int2> pairs(nn);
+        int32_t cnt[8] = {0};
+        auto eighth = [&](int32_t v) { return (int)(((int64_t)v * 8) / n); };
+        for (int32_t o = 0; o < n; ++o) ++cnt[eighth(inv[(size_t)o])];
+        g->unperm_off[0] = 0;
+        for (int x = 0; x < 8; ++x) g->unperm_off[x + 1] = g->unperm_off[x] + cnt[x];
+        int32_t pos[8];
+        for (int x = 0; x < 8; ++x) pos[x] = g->unperm_off[x];
+        for (int32_t o = 0; o < n; ++o) {
+            const int32_t v = inv[(size_t)o];
+            pairs[(size_t)pos[eighth(v)]++] = make_int2(o, v);
+        }
+        if ((rc = dev_alloc(g, &g->d_unperm, nn, "vertex order")) != MM_OK) return fail(rc);
         if ((rc = dev_alloc(g, &g->d_match_out, nn, "match")) != MM_OK) return fail(rc);
-        if (hipMemcpy(g->d_new_of, inv.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess)
+        if (hipMemcpy(g->d_unperm, pairs.data(), sizeof(int2) * nn, hipMemcpyHostToDevice) != hipSuccess)
             return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
     }
     *out = g;
@@ -1346,11 +1383,25 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.st = g->d_st;
     d.resp = g->d_resp;
     d.lists = g->d_lists;
-    d.ctr = g->d_ctr;
+    // This run's counter set; its init clears the other set for the next run,
+    // so the next run switches sets once this run's init has been submitted
+    // (whatever happens after that).
+    const int cset = g->ctr_set;
+    struct SetFlip {
+        mm_graph *g;
+        bool init_submitted = false;
+        ~SetFlip()
+        {
+            if (init_submitted) g->ctr_set ^= 1;
+        }
+    } flip{g};
+    d.ctr = g->d_ctr + (size_t)cset * ROUND_SLOTS * CTR_STRIDE;
+    d.ctr_next = g->d_ctr + (size_t)(cset ^ 1) * ROUND_SLOTS * CTR_STRIDE;
     d.ctrb = g->d_ctrb;
     d.ctr64 = g->d_ctr64;
     d.ctrb64 = g->d_ctrb64;
-    d.hub64 = g->d_hub64;
+    d.hub64 = g->d_hub64 + (size_t)cset * ROUND_SLOTS * 2;
+    d.hub64_next = g->d_hub64 + (size_t)(cset ^ 1) * ROUND_SLOTS * 2;
     d.seeds = g->d_seeds;
     d.loc = g->d_loc;
     d.vround = g->d_vround;
@@ -1401,10 +1452,9 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     // maximal runs also total the counters of rounds [r0, r1 + 1].
     auto enqueue = [&](int r0, int r1) -> int32_t {
         if (r0 == 1) {
-            MM_LAUNCH(K_CLEAR, hipLaunchKernelGGL(k_clear, dim3(8), dim3(256), 0, g->stream, g->d_ctr, g->d_hub64,
-                                                  ROUND_SLOTS * CTR_STRIDE, ROUND_SLOTS * 2));
             if (c.PUSH) MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select<true>, dim3(c.nb), dim3(block), 0, g->stream, d));
             else MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select<false>, dim3(c.nb), dim3(block), 0, g->stream, d));
+            if (!use_graph) flip.init_submitted = true;
         }
         for (int rr = r0; rr <= r1; ++rr) {
             const int do_select = rr < R_run;
@@ -1414,6 +1464,8 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
             else MM_LAUNCH(K_MATCH, hipLaunchKernelGGL(k_match_select<false>, dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
         }
         if (maximal) return enq_reduce(r0, std::min(r1 + 1, R_run));
+        // A fixed run ends with the result in the caller's order, in the same sequence.
+        if (r1 == R_run) mm_graph_enqueue_unpermute(g);
         return MM_OK;
     };
     // Replays the captured launch sequence of rounds [r0, r1], capturing it on first use.
@@ -1421,7 +1473,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     std::memset(&key, 0, sizeof(key));
     key.R_run = R_run; key.W = c.W; key.A8 = c.A8; key.PUSH = c.PUSH; key.G = c.G; key.U = c.U; key.block = c.block;
     key.hub_deg = c.hub_deg; key.nb = c.nb; key.bcap = c.bcap; key.maximal = maximal;
-    key.hub_blocks = c.hub_blocks; key.barrier = o->barrier; key.count_nnz = d.count_nnz;
+    key.hub_blocks = c.hub_blocks; key.barrier = o->barrier; key.count_nnz = d.count_nnz; key.cset = cset;
     auto run_chunk = [&](int r0, int r1) -> int32_t {
         if (!use_graph) return enqueue(r0, r1);
         key.r0 = r0;
@@ -1429,6 +1481,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
         for (const CachedExec &ce : g->execs)
             if (std::memcmp(&ce.key, &key, sizeof(key)) == 0) {
                 MM_HIP_TRY(hipGraphLaunch(ce.exec, g->stream), "hipGraphLaunch");
+                if (r0 == 1) flip.init_submitted = true;
                 return MM_OK;
             }
         hipGraph_t graph = nullptr;
@@ -1448,6 +1501,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
         }
         g->execs.push_back(ce);
         MM_HIP_TRY(hipGraphLaunch(ce.exec, g->stream), "hipGraphLaunch");
+        if (r0 == 1) flip.init_submitted = true;
         return MM_OK;
     };
 
@@ -1461,7 +1515,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
             MM_HIP_TRY(hipGetLastError(), "A HIP error occurred during the matching process");
             if (maximal) {
                 const int k_hi = std::min(end + 1, R_run);
-                MM_HIP_TRY(hipMemcpyAsync(g->h_ctr + (size_t)r * CTR_STRIDE, g->d_ctr + (size_t)r * CTR_STRIDE,
+                MM_HIP_TRY(hipMemcpyAsync(g->h_ctr + (size_t)r * CTR_STRIDE, d.ctr + (size_t)r * CTR_STRIDE,
                                           sizeof(int32_t) * (size_t)(k_hi - r + 1) * CTR_STRIDE,
                                           hipMemcpyDeviceToHost, g->stream),
                            "Unable to read round counters");
@@ -1479,14 +1533,15 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
         return mm::set_error(MM_ERR_SEEDS, "mm_match: ran out of seeds before the matching converged");
     if (g->n == 0) rounds = maximal ? 1 : R_max;
     if (rounds < 0) rounds = R_max;
-    // The result in the caller's numbering is part of the matching.
-    mm_graph_enqueue_unpermute(g);
+    // The result in the caller's numbering is part of the matching (fixed runs
+    // enqueue it with their last round).
+    if (maximal) mm_graph_enqueue_unpermute(g);
     MM_HIP_TRY(hipEventRecord(g->ev_t2, g->stream), "hipEventRecord");
     // Totals of every executed round (a maximal run's last, empty round has none).
     const int r_stats = g->n > 0 ? std::min(rounds, R_run) : 0;
     if (stats && r_stats > 0) {
         if ((rc = enq_reduce(1, r_stats)) != MM_OK) return rc;
-        MM_HIP_TRY(hipMemcpyAsync(g->h_ctr, g->d_ctr, sizeof(int32_t) * ROUND_SLOTS * CTR_STRIDE,
+        MM_HIP_TRY(hipMemcpyAsync(g->h_ctr, d.ctr, sizeof(int32_t) * ROUND_SLOTS * CTR_STRIDE,
                                   hipMemcpyDeviceToHost, g->stream),
                    "Unable to read round counters");
         MM_HIP_TRY(hipMemcpyAsync(g->h_ctr64, g->d_ctr64, sizeof(unsigned long long) * ROUND_SLOTS * CTR64_STRIDE,
@@ -1655,9 +1710,14 @@ int32_t mm_graph_alloc(mm_graph *g, void **p, size_t bytes, const char *what)
 
 void mm_graph_enqueue_unpermute(mm_graph *g)
 {
-    if (g->relabelled && g->n > 0)
-        hipLaunchKernelGGL(k_unpermute, dim3(std::min<int64_t>(((int64_t)g->n + 255) / 256, (int64_t)g->num_cu * 8)),
-                           dim3(256), 0, g->stream, g->d_match, g->d_new_of, g->d_match_out, g->n);
+    if (g->relabelled && g->n > 0) {
+        UnpermOff off;
+        std::memcpy(off.off, g->unperm_off, sizeof(off.off));
+        const int per_xcd = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)g->n / 8 + 255) / 256,
+                                                                          (int64_t)std::max(1, g->num_cu / 8) * 4));
+        hipLaunchKernelGGL(k_unpermute, dim3(8 * per_xcd), dim3(256), 0, g->stream, (const int2 *)g->d_unperm, off,
+                           (const int32_t *)g->d_match, g->d_match_out);
+    }
 }
 
 extern "C" {

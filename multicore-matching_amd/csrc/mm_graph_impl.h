@@ -12,7 +12,7 @@
 
 // Key of a captured launch sequence: every value baked into its kernel arguments.
 struct ExecKey {
-    int32_t R_run, W, A8, PUSH, G, U, block, hub_deg, nb, bcap, maximal, hub_blocks, count_nnz, r0, r1;
+    int32_t R_run, W, A8, PUSH, G, U, block, hub_deg, nb, bcap, maximal, hub_blocks, count_nnz, r0, r1, cset;
     uint32_t barrier;
 };
 struct CachedExec {
@@ -53,13 +53,15 @@ struct mm_graph {
     // batched graphs
     int32_t n_graphs = 0;
     int32_t *d_loc = nullptr;        // graph-local original id per internal vertex
-    int32_t *d_new_of = nullptr;     // internal id per original (global) id, when relabelled
+    int2 *d_unperm = nullptr;        // {original position, internal id} pairs, grouped by eighth of internal ids
+    int32_t unperm_off[9] = {0};     // group offsets in d_unperm
     int32_t *d_match_out = nullptr;  // result in the caller's numbering, when relabelled
     bool relabelled = false;
     int32_t *d_voff = nullptr;
     uint8_t *d_vround = nullptr;
     int32_t *d_grounds = nullptr;
     std::vector<uint32_t> last_seeds;  // seed table currently on the device
+    int ctr_set = 0;                   // counter set of the next run (runs alternate between two)
     std::vector<CachedExec> execs;     // captured launch sequences
     hipEvent_t ev_t1 = nullptr, ev_t2 = nullptr;
     std::vector<hipEvent_t> prof_ev;

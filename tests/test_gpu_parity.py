@@ -224,3 +224,24 @@ def test_graph_replays_equal_eager_launches():
         assert oracle.test_matching(csr, mid.match)[0]
         rep = g.match(kind=MM_WEIGHTED, maximal=False, max_rounds=5, seeds=SEEDS)
         assert np.array_equal(rep.match, eager_fix.match), i
+
+
+def test_consecutive_runs_alternate_counter_sets_without_leaking_state():
+    """Runs switch between two counter sets; mixing eager, profiled and
+    replayed runs, fixed and maximal, with and without hub rows must keep every
+    result and every round count exact."""
+    csr = mmamd.gen_rmat(13, seed=77, weighted=True)
+    exp_max = oracle.match(csr, True, True, SEEDS, nthreads=0)
+    exp_fix = oracle.match(csr, True, False, SEEDS, max_rounds=4, nthreads=0)
+    g = Graph(csr)
+    modes = [dict(), dict(graph_mode=1), dict(profile=True), dict(hub_degree=8), dict(hub_degree=8, graph_mode=1)]
+    for k in range(12):
+        kw = modes[k % len(modes)]
+        maximal = k % 3 != 2
+        say("run", k, kw, "maximal", maximal)
+        res = g.match(kind=MM_WEIGHTED, maximal=maximal, max_rounds=0 if maximal else 4, seeds=SEEDS, **kw)
+        exp, R, act, mat = exp_max if maximal else exp_fix
+        assert res.rounds == R
+        np.testing.assert_array_equal(res.match, exp)
+        np.testing.assert_array_equal(res.active, act)
+        np.testing.assert_array_equal(res.matched, mat)
