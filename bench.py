@@ -110,6 +110,41 @@ def cpu_baseline(csr, seeds, r99: int, t_budget_s: float = 20.0) -> dict:
                       f"TBB weighted round matcher (type 11), {reps} timed runs, {t * 1e3:.1f} ms each"}
 
 
+def cpu_matchers(csr, seeds, r99: int) -> dict:
+    """The reference's CPU matchers on the same graph, restated in the oracle
+    and timed on this host (a reported comparison, never the measured path):
+    types 0 (random greedy), 4 (weighted vertex-order greedy) and 5 (edge-sorted
+    greedy, src/matchcpu.cu:259-303, 695-810) single-threaded, and the TBB
+    round matcher restated with OpenMP (type 11 maximal, src/matchtbb.cu:567-623)
+    on 1..all cores for the first r99 rounds plus one full maximal run."""
+    import oracle
+
+    out = {}
+    for name, fn in (("type0_random_greedy", oracle.cpu_random), ("type4_weighted_greedy", oracle.cpu_weighted),
+                     ("type5_edge_sorted_greedy", oracle.cpu_weighted_edge)):
+        t0 = time.perf_counter()
+        m = fn(csr)
+        dt = time.perf_counter() - t0
+        w, size = mmamd.get_weight(csr, m)
+        out[name] = {"matched": size, "weight": w, "ms": round(dt * 1e3, 2), "cores": 1}
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    m, R, _, mat = oracle.match(csr, True, True, seeds, nthreads=threads)
+    dt = time.perf_counter() - t0
+    w, size = mmamd.get_weight(csr, m)
+    out["type11_rounds_maximal"] = {"rounds": R, "matched": size, "weight": w, "ms": round(dt * 1e3, 2),
+                                    "cores": threads}
+    sweep = {}
+    k = 1
+    while k <= threads:
+        t0 = time.perf_counter()
+        oracle.match(csr, True, False, seeds, max_rounds=r99, nthreads=k)
+        sweep[str(k)] = round((time.perf_counter() - t0) * 1e3, 2)
+        k = k * 2 if k * 2 <= threads or k == threads else threads
+    out["type11_to_t99_ms_by_threads"] = sweep
+    return out
+
+
 def tiles_of(t: dict) -> dict:
     return {"lanes": t["lanes_per_vertex"], "block_size": t["block_size"], "hub_degree": t["hub_degree"],
             "unroll": t["entries_per_thread"]}
@@ -226,6 +261,8 @@ def run_single(args, csr, name: str, workload_desc: str) -> dict:
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         line["cpu_baseline"] = cpu_baseline(csr, seeds, r99)
+        line["cpu_matchers"] = cpu_matchers(csr, seeds, r99)
+        log(f"cpu matchers: {json.dumps(line['cpu_matchers'])}")
     g.close()
     return line
 

@@ -37,6 +37,41 @@ class GraphMatching
         const Graph &graph;
 };
 
+// Sequential CPU matchers (src/matchcpu.h:50-108): type 0 random greedy,
+// type 4 weighted vertex-order greedy, type 5 edge-sorted greedy.  The minimum
+// degree and Karp--Sipser matchers (types 1-3) are not part of this build.
+class GraphMatchingCPURandom : public GraphMatching
+{
+    public:
+        GraphMatchingCPURandom(const Graph &);
+        ~GraphMatchingCPURandom();
+
+        void performMatching(std::vector<int> &, hipEvent_t &, hipEvent_t &) const;
+};
+
+class GraphMatchingCPUWeighted : public GraphMatching
+{
+    public:
+        GraphMatchingCPUWeighted(const Graph &);
+        ~GraphMatchingCPUWeighted();
+
+        void performMatching(std::vector<int> &, hipEvent_t &, hipEvent_t &) const;
+};
+
+class GraphMatchingCPUWeightedEdge : public GraphMatching
+{
+    public:
+        GraphMatchingCPUWeightedEdge(const Graph &);
+        ~GraphMatchingCPUWeightedEdge();
+
+        void performMatching(std::vector<int> &, hipEvent_t &, hipEvent_t &) const;
+};
+
+// Records ev on the null stream and waits for it when a HIP device is
+// present; the CPU matchers time themselves with these events like the
+// reference (src/matchcpu.cu:266-267), and skip them without a device.
+void recordHostEvent(hipEvent_t &ev);
+
 }  // namespace mtc
 
 #endif

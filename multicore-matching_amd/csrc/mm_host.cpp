@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <atomic>
 #include <cstdlib>
 #include <string>
 #include <thread>
@@ -14,8 +15,12 @@
 namespace {
 thread_local std::string g_last_error;
 
+std::atomic<unsigned> g_host_threads{0};
+
 unsigned n_workers()
 {
+    const unsigned forced = g_host_threads.load();
+    if (forced) return forced > 64 ? 64 : forced;
     const char *env = std::getenv("MM_GEN_THREADS");
     unsigned t = env ? (unsigned)std::atoi(env) : std::thread::hardware_concurrency();
     if (t == 0) t = 1;
@@ -43,6 +48,8 @@ int32_t set_errorf(int32_t code, const char *fmt, ...)
 }
 
 void clear_error() { g_last_error.clear(); }
+
+void set_host_threads(unsigned n) { g_host_threads.store(n); }
 
 void parallel_for(int64_t total, const std::function<void(int64_t, int64_t, unsigned)> &body)
 {

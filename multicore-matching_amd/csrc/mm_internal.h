@@ -12,9 +12,11 @@ int32_t set_error(int32_t code, const char *msg);
 int32_t set_errorf(int32_t code, const char *fmt, ...);
 void clear_error();
 
-// Static partition of [0, total) over host worker threads (MM_GEN_THREADS or
-// all hardware threads); body(begin, end, worker).
+// Static partition of [0, total) over host worker threads (set_host_threads,
+// else MM_GEN_THREADS, else all hardware threads); body(begin, end, worker).
 void parallel_for(int64_t total, const std::function<void(int64_t, int64_t, unsigned)> &body);
+// Worker count of parallel_for; 0 restores the default.
+void set_host_threads(unsigned n);
 
 }  // namespace mm
 
