@@ -99,18 +99,23 @@ struct Stamp {
     bool dev = false;
     hipEvent_t ev = nullptr;
     chrono::steady_clock::time_point host;
+    // Every HIP call sits in a RandStreamGuard: the runtime may draw from
+    // rand(), which would move the seeds the matchers take from it.
     explicit Stamp(bool d) : dev(d)
     {
+        RandStreamGuard guard;
         if (dev && hipEventCreate(&ev) != hipSuccess) ev = nullptr;
     }
     ~Stamp()
     {
+        RandStreamGuard guard;
         if (ev) (void)hipEventDestroy(ev);
     }
     void mark()
     {
         host = chrono::steady_clock::now();
         if (ev) {
+            RandStreamGuard guard;
             (void)hipEventRecord(ev, 0);
             (void)hipEventSynchronize(ev);
         }
@@ -118,6 +123,7 @@ struct Stamp {
     static float ms(const Stamp &a, const Stamp &b)
     {
         float t = 0.0f;
+        RandStreamGuard guard;
         if (a.ev && b.ev && hipEventElapsedTime(&t, a.ev, b.ev) == hipSuccess) return t;
         return chrono::duration<float, milli>(b.host - a.host).count();
     }
@@ -130,12 +136,15 @@ bool takesValue(const string &a)
 }
 
 // Per-round weight and size of one GPU type: the rounds are deterministic in
-// the seed table, so round r's state is that of an r-round run.
+// the seed table, so round r's state is that of an r-round run.  Returns the
+// rounds executed (the seeds the caller then consumes) or -1.
 int intermediate(const Graph &graph, int type, unsigned int barrier, int device)
 {
     const bool weighted = type >= 8;
     const bool maximal = type == 7 || type == 9;
     const bool has_w = graph.neighbourWeights.size() == graph.neighbours.size() && !graph.neighbours.empty();
+    vector<unsigned int> seeds = peekRand(MM_NR_MAX_MATCH_ROUNDS);
+    RandStreamGuard guard;  // the library's HIP calls below must not move the stream
     mm_graph *g = nullptr;
     if (mm_graph_create(device, graph.nrVertices, (int64_t)graph.neighbours.size(),
                         reinterpret_cast<const int32_t *>(graph.neighbourRanges.data()), graph.neighbours.data(),
@@ -143,7 +152,6 @@ int intermediate(const Graph &graph, int type, unsigned int barrier, int device)
         cerr << mm_last_error() << endl;
         return -1;
     }
-    vector<unsigned int> seeds = peekRand(MM_NR_MAX_MATCH_ROUNDS);
     mm_opts o;
     memset(&o, 0, sizeof(o));
     o.kind = weighted ? MM_WEIGHTED : MM_RANDOM;
@@ -174,8 +182,7 @@ int intermediate(const Graph &graph, int type, unsigned int barrier, int device)
         cout << r << "\t" << weight << "\t" << size << endl;
     }
     mm_graph_destroy(g);
-    for (int r = 0; r < R; ++r) (void)rand();
-    return 0;
+    return R;
 }
 
 }  // namespace
@@ -295,7 +302,9 @@ int main(int argc, char **argv)
             for (int k = 0; k < repetitions; ++k) {
                 if (randomise) graph2.random_shuffle();
                 if (interm && type >= 6 && type <= 9) {
-                    if (intermediate(graph2, type, barrier, device) != 0) return -1;
+                    const int R = intermediate(graph2, type, barrier, device);
+                    if (R < 0) return -1;
+                    for (int r = 0; r < R; ++r) (void)rand();  // one seed per executed round
                     continue;
                 }
                 Stamp s0(haveDevice), s1(haveDevice), s2(haveDevice), s3(haveDevice);

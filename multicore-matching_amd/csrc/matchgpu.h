@@ -78,6 +78,23 @@ class GraphMatchingGPUWeightedMaximal : public GraphMatchingGPU
 // Reads the next `count` values of glibc's rand() stream without consuming them.
 std::vector<unsigned int> peekRand(int count);
 
+// Keeps the process-wide rand() stream out of reach of the HIP runtime, which
+// may draw from it: the stream's state is copied on construction and put back
+// on destruction.  Every HIP call a driver makes between srand() and the last
+// rand() it needs belongs inside one (the matcher classes use it internally).
+class RandStreamGuard
+{
+    public:
+        RandStreamGuard();
+        ~RandStreamGuard();
+        RandStreamGuard(const RandStreamGuard &) = delete;
+        RandStreamGuard &operator=(const RandStreamGuard &) = delete;
+
+    private:
+        char saved[256];
+        size_t bytes;
+};
+
 }  // namespace mtc
 
 #endif

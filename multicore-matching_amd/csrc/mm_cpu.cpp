@@ -16,6 +16,7 @@
 
 #include "graph.h"
 #include "matchcpu.h"
+#include "matchgpu.h"
 #include "matchtbb.h"
 #include "mm_hash.h"
 #include "mm_internal.h"
@@ -27,6 +28,7 @@ namespace mtc
 
 void recordHostEvent(hipEvent_t &ev)
 {
+    RandStreamGuard guard;  // the HIP runtime must not move the caller's rand() stream
     int n = 0;
     if (ev && hipGetDeviceCount(&n) == hipSuccess && n > 0) {
         (void)hipEventRecord(ev, 0);

@@ -125,48 +125,38 @@ vector<unsigned int> peekRand(int count)
     return out;
 }
 
+// Copies the state on entry and puts it back on exit, so whatever the runtime
+// draws in between, the caller's stream continues exactly where it was.
 namespace
 {
-
-// Keeps the process-wide rand() stream out of reach of the HIP runtime: the
-// state (buffer and read position) is copied on entry and put back on exit,
-// so whatever the runtime draws while the matcher calls into it, the caller's
-// stream continues exactly where it was, as with the reference.
-class RandStreamGuard
+size_t stateBytes(const char *state)
 {
-    public:
-        RandStreamGuard()
-        {
-            static char scratch[256];
-            char *prev = initstate(1u, scratch, sizeof(scratch));
-            bytes = state_bytes(prev);
-            std::memcpy(saved, prev, bytes);
-            setstate(prev);
-        }
-        ~RandStreamGuard()
-        {
-            // Restore into the spare of two buffers: setstate() writes the read
-            // position of the buffer it leaves, which must not be the copy.
-            static char live[2][256];
-            static int which = 0;
-            which ^= 1;
-            std::memcpy(live[which], saved, bytes);
-            setstate(live[which]);
-        }
-
-    private:
-        static size_t state_bytes(const char *state)
-        {
-            static const int degrees[5] = {0, 7, 15, 31, 63};
-            const int32_t hdr = *reinterpret_cast<const int32_t *>(state);
-            const int type = hdr % 5;
-            return sizeof(int32_t) * (size_t)(degrees[type < 0 ? 0 : type] + 1);
-        }
-        char saved[256];
-        size_t bytes;
-};
-
+    static const int degrees[5] = {0, 7, 15, 31, 63};
+    const int32_t hdr = *reinterpret_cast<const int32_t *>(state);
+    const int type = hdr % 5;
+    return sizeof(int32_t) * (size_t)(degrees[type < 0 ? 0 : type] + 1);
+}
 }  // namespace
+
+RandStreamGuard::RandStreamGuard()
+{
+    static char scratch[256];
+    char *prev = initstate(1u, scratch, sizeof(scratch));
+    bytes = stateBytes(prev);
+    std::memcpy(saved, prev, bytes);
+    setstate(prev);
+}
+
+RandStreamGuard::~RandStreamGuard()
+{
+    // Restore into the spare of two buffers: setstate() writes the read
+    // position of the buffer it leaves, which must not be the copy.
+    static char live[2][256];
+    static int which = 0;
+    which ^= 1;
+    std::memcpy(live[which], saved, bytes);
+    setstate(live[which]);
+}
 
 GraphMatchingGPU::GraphMatchingGPU(const Graph &_graph, const int &_threadsPerBlock, const unsigned int &_selectBarrier)
     : GraphMatching(_graph), threadsPerBlock(_threadsPerBlock), selectBarrier(_selectBarrier), dgraph(nullptr),

@@ -126,6 +126,7 @@ struct LDev {
     float *sx, *sy, *sw;
     int32_t *sv, *sr;
     unsigned long long *scount;
+    unsigned long long *resp;  // PUSH proposal keys, one per vertex
 };
 
 // Row entry j in matching order: weight-sorted pairs (id complemented when the
@@ -194,14 +195,16 @@ __device__ __forceinline__ void group_min2(int &p1, int &p2)
 
 // Select of round r over tiles of LTILE vertices.  Phase 1: groups of G lanes
 // scan the rows of the tile's active vertices (coalesced chunks of G entries)
-// for the features, into LDS.  Phase 2: one thread per vertex evaluates the
-// network and draws the colour.  The network parameters arrive as their own
+// for the unmatched / previously-red neighbour counts and the positions of the
+// first two candidates in row order, into LDS.  Phase 2: one thread per vertex
+// reads those two entries, hashes, evaluates the network and draws the
+// colour; PUSH rounds also clear the vertex's proposal key here, before the
+// request phase posts to it.  The network parameters arrive as their own
 // read-only, non-aliased argument, so their wave-uniform loads are scalar.
-template <bool A8, int G>
+template <bool A8, bool PUSH, int G>
 __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed, const float *__restrict__ params)
 {
-    __shared__ float fs[LTILE][NF + 1];
-    __shared__ int8_t fl[LTILE];  // -1 inactive, 0 active, 1 active without unmatched neighbours
+    __shared__ int32_t s_p1[LTILE], s_p2[LTILE], s_alive[LTILE], s_redp[LTILE];
     const int t = threadIdx.x;
     const int gl = t & (G - 1);
     const int grp = t / G;
@@ -214,10 +217,7 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
             for (int k = grp; k < LTILE; k += GROUPS) {
                 const int64_t v = base + k;
                 const int32_t m = v < d.n ? d.match[v] : 4;
-                if (m >= 2) {
-                    if (gl == 0) fl[k] = -1;
-                    continue;
-                }
+                if (m >= 2) continue;
                 const int2 rg = d.ranges[v];
                 int alive = 0, redp = 0, p1 = INT_MAX, p2 = INT_MAX;
                 for (int32_t j = rg.x + gl; j < rg.y; j += G) {
@@ -238,28 +238,10 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
                 redp = group_sum<G>(redp);
                 group_min2<G>(p1, p2);
                 if (gl == 0) {
-                    float w1 = 0.0f, w2 = 0.0f, bit = 0.0f;
-                    if (p1 != INT_MAX) {
-                        int32_t u;
-                        bool valid;
-                        l_entry<A8>(d, p1, u, valid, w1);
-                        const uint32_t hv = mm_select_hash((uint32_t)d.loc[v], seed);
-                        bit = hv < mm_select_hash((uint32_t)d.loc[u], seed) ? 1.0f : 0.0f;
-                    }
-                    if (p2 != INT_MAX) {
-                        int32_t u;
-                        bool valid;
-                        l_entry<A8>(d, p2, u, valid, w2);
-                    }
-                    fs[k][0] = (float)min(rg.y - rg.x, 1024) * 0.015625f;
-                    fs[k][1] = w1;
-                    fs[k][2] = w2;
-                    fs[k][3] = w1 > 0.0f ? (float)((double)w2 / (double)w1) : 0.0f;
-                    fs[k][4] = (float)min(alive, 1024) * 0.015625f;
-                    fs[k][5] = (float)min(redp, 1024) * 0.015625f;
-                    fs[k][6] = (float)(r - 1) * 0.03125f;
-                    fs[k][7] = bit;
-                    fl[k] = alive == 0 ? 1 : 0;
+                    s_p1[k] = p1;
+                    s_p2[k] = p2;
+                    s_alive[k] = alive;
+                    s_redp[k] = redp;
                 }
             }
             __syncthreads();
@@ -270,28 +252,51 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
         int c = LC_RED;
         if (act) {
             const uint32_t vl = (uint32_t)d.loc[v];
-            const int plain = mm_select_hash(vl, seed) < d.barrier ? LC_BLUE : LC_RED;
-            if (!net) {
-                c = plain;
-            } else if (fl[t] == 1) {
-                c = LC_RETIRE;
-            } else {
+            const uint32_t hv = mm_select_hash(vl, seed);
+            const int plain = hv < d.barrier ? LC_BLUE : LC_RED;
+            c = plain;
+            if (scan) {
+                const int2 rg = d.ranges[v];
+                const int32_t p1 = s_p1[t], p2 = s_p2[t], alive = s_alive[t];
+                float w1 = 0.0f, w2 = 0.0f, bit = 0.0f;
+                if (p1 != INT_MAX) {
+                    int32_t u;
+                    bool valid;
+                    l_entry<A8>(d, p1, u, valid, w1);
+                    bit = hv < mm_select_hash((uint32_t)d.loc[u], seed) ? 1.0f : 0.0f;
+                }
+                if (p2 != INT_MAX) {
+                    int32_t u;
+                    bool valid;
+                    l_entry<A8>(d, p2, u, valid, w2);
+                }
                 float x[NF];
+                x[0] = (float)min(rg.y - rg.x, 1024) * 0.015625f;
+                x[1] = w1;
+                x[2] = w2;
+                x[3] = w1 > 0.0f ? (float)((double)w2 / (double)w1) : 0.0f;
+                x[4] = (float)min(alive, 1024) * 0.015625f;
+                x[5] = (float)min(s_redp[t], 1024) * 0.015625f;
+                x[6] = (float)(r - 1) * 0.03125f;
+                x[7] = bit;
+                if (d.trace) {
 #pragma unroll
-                for (int k = 0; k < NF; ++k) x[k] = fs[t][k];
-                const float z = mlp_z(params, x);
-                if (!isfinite(z)) {
-                    c = plain;
-                } else {
-                    const uint32_t T = blue_threshold(z);
-                    c = (mm_select_hash(vl, seed ^ MM_LEARN_DRAW_TWEAK) >> 16) < T ? LC_BLUE : LC_RED;
+                    for (int k = 0; k < NF; ++k) d.feat[(size_t)v * NF + k] = x[k];
+                }
+                if (net) {
+                    if (alive == 0) {
+                        c = LC_RETIRE;
+                    } else {
+                        const float z = mlp_z(params, x);
+                        if (isfinite(z)) {
+                            const uint32_t T = blue_threshold(z);
+                            c = (mm_select_hash(vl, seed ^ MM_LEARN_DRAW_TWEAK) >> 16) < T ? LC_BLUE : LC_RED;
+                        }
+                    }
                 }
             }
             d.col[v] = (uint8_t)c;
-            if (d.trace) {
-#pragma unroll
-                for (int k = 0; k < NF; ++k) d.feat[(size_t)v * NF + k] = fs[t][k];
-            }
+            if (PUSH) d.resp[v] = 0ull;
         }
         n_act += act;
         n_blue += act && c == LC_BLUE;
@@ -305,6 +310,23 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
     }
 }
 
+// Proposal key of a PUSH request: order-preserving image of the weight (-0
+// folded onto +0) above the complement of the proposer's graph-local id, so
+// the largest key is the heaviest proposer, the smallest id on ties -- the
+// first in an ascending row, as the reference's respond scan picks it.
+__device__ __forceinline__ unsigned long long l_key(float w, uint32_t vl)
+{
+    const uint32_t bits = __float_as_uint(w == 0.0f ? 0.0f : w);
+    const uint32_t hi = (bits & 0x80000000u) ? ~bits : (bits | 0x80000000u);
+    return ((unsigned long long)hi << 32) | (unsigned long long)(0xFFFFFFFFu - vl);
+}
+
+__device__ __forceinline__ float l_key_weight(unsigned long long k)
+{
+    const uint32_t hi = (uint32_t)(k >> 32);
+    return __uint_as_float((hi & 0x80000000u) ? (hi & 0x7FFFFFFFu) : ~hi);
+}
+
 // Bits of this lane's group in a wavefront ballot, as a G-bit mask.
 template <int G>
 __device__ __forceinline__ unsigned long long group_bits(unsigned long long ballot)
@@ -315,7 +337,7 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long ball
 
 // Request: each blue vertex's group scans its row in matching order, G
 // entries at a time, and stops at the first red unmatched neighbour.
-template <bool A8, int G>
+template <bool A8, bool PUSH, int G>
 __global__ void k_l_request(LDev d)
 {
     const int gl = __lane_id() & (G - 1);
@@ -360,6 +382,7 @@ __global__ void k_l_request(LDev d)
         if (blue && gl == 0) {
             d.req[v] = found ? best : (any_alive ? LREQ_NONE : LREQ_DEAD);
             d.reqw[v] = found ? bw : 0.0f;
+            if (PUSH && found) atomicMax(d.resp + best, l_key(bw, (uint32_t)d.loc[v]));
         }
     }
 }
@@ -404,6 +427,7 @@ __global__ void k_l_respond(LDev d)
     }
 }
 
+template <bool PUSH>
 __global__ void k_l_match(LDev d, int r, uint32_t seed)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -420,9 +444,28 @@ __global__ void k_l_match(LDev d, int r, uint32_t seed)
             c = d.col[v];
             const int32_t q = d.req[v];
             vl = (uint32_t)d.loc[v];
-            if (c == LC_RETIRE || q == LREQ_DEAD) nm = 2;
-            else if (q >= 0 && d.req[q] == v) nm = 4 + (int32_t)min(vl, (uint32_t)d.loc[q]);
-            else nm = c;
+            if (c == LC_RETIRE || q == LREQ_DEAD) {
+                nm = 2;
+            } else if (PUSH) {
+                // red: the winning key names its proposer; blue: it won iff its target's key names it
+                if (c == LC_RED) {
+                    const unsigned long long k = d.resp[v];
+                    if (k != 0ull) {
+                        nm = 4 + (int32_t)min(vl, 0xFFFFFFFFu - (uint32_t)k);
+                        d.reqw[v] = l_key_weight(k);
+                    } else {
+                        nm = c;
+                    }
+                } else if (q >= 0 && (uint32_t)d.resp[q] == 0xFFFFFFFFu - vl) {
+                    nm = 4 + (int32_t)min(vl, (uint32_t)d.loc[q]);
+                } else {
+                    nm = c;
+                }
+            } else if (q >= 0 && d.req[q] == v) {
+                nm = 4 + (int32_t)min(vl, (uint32_t)d.loc[q]);
+            } else {
+                nm = c;
+            }
             d.match[v] = nm;
         }
         n_mat += act && nm >= 4;
@@ -712,6 +755,9 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         return mm::set_error(MM_ERR_INVALID, "mm_match_learned: trace_keep above 65536");
     const bool use_mlp = lo && lo->params;
     const bool A8 = o->kind == MM_WEIGHTED && g->weighted;
+    // Proposals by atomic max where that equals the reference's respond scan
+    // (symmetric CSR, ascending rows, symmetric weights), as in mm_match.
+    const bool push = (A8 ? g->push_ok_weights : g->push_ok_struct) && g->d_resp != nullptr;
     const size_t nn = (size_t)g->n;
 
     ML_TRY(hipSetDevice(g->device), "hipSetDevice");
@@ -748,6 +794,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
     d.ctr = g->d_lctr;
     d.part = g->d_lpart;
     d.scount = g->d_lsamp;
+    d.resp = g->d_resp;
     if (trace) {
         const size_t cap = (size_t)lo->trace_cap;
         if ((rc = tb.alloc(&d.sx, cap * NF, "trace features")) != MM_OK || (rc = tb.alloc(&d.sy, cap, "trace")) != MM_OK ||
@@ -775,18 +822,19 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         const int end = std::min(R_run, r + chunk - 1);
         for (int rr = r; rr <= end; ++rr) {
             const uint32_t seed = o->seeds[rr - 1];
-            if (A8) {
-                hipLaunchKernelGGL((k_l_select<true, SEL_G>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed,
-                                   (const float *)g->d_params);
-                hipLaunchKernelGGL((k_l_request<true, SCAN_G>), dim3(grid), dim3(block), 0, st, d);
-                hipLaunchKernelGGL((k_l_respond<true, SCAN_G>), dim3(grid), dim3(block), 0, st, d);
-            } else {
-                hipLaunchKernelGGL((k_l_select<false, SEL_G>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed,
-                                   (const float *)g->d_params);
-                hipLaunchKernelGGL((k_l_request<false, SCAN_G>), dim3(grid), dim3(block), 0, st, d);
-                hipLaunchKernelGGL((k_l_respond<false, SCAN_G>), dim3(grid), dim3(block), 0, st, d);
-            }
-            hipLaunchKernelGGL(k_l_match, dim3(grid), dim3(block), 0, st, d, rr, seed);
+            const float *P = (const float *)g->d_params;
+#define ML_ROUND(A8_, PUSH_)                                                                                   \
+    do {                                                                                                       \
+        hipLaunchKernelGGL((k_l_select<A8_, PUSH_, SEL_G>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed, P); \
+        hipLaunchKernelGGL((k_l_request<A8_, PUSH_, SCAN_G>), dim3(grid), dim3(block), 0, st, d);             \
+        if (!PUSH_) hipLaunchKernelGGL((k_l_respond<A8_, SCAN_G>), dim3(grid), dim3(block), 0, st, d);         \
+        hipLaunchKernelGGL((k_l_match<PUSH_>), dim3(grid), dim3(block), 0, st, d, rr, seed);                  \
+    } while (0)
+            if (A8 && push) ML_ROUND(true, true);
+            else if (A8) ML_ROUND(true, false);
+            else if (push) ML_ROUND(false, true);
+            else ML_ROUND(false, false);
+#undef ML_ROUND
             hipLaunchKernelGGL(k_l_reduce, dim3(1), dim3(64 * LCTR), 0, st, (const int32_t *)d.part, sgrid, grid,
                                d.ctr + (size_t)rr * LCTR);
         }

@@ -109,6 +109,9 @@ def compare(csr, params, seeds=None, graph=None, max_rounds: int = 0) -> dict:
     if seeds is None:
         seeds = mmamd.reference_seeds(mmamd.MM_MAX_ROUNDS)
     g = graph if graph is not None else mmamd.Graph(csr)
+    # untimed first runs: buffers and first kernel launches
+    g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
+    g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
     plain = g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
     learned = g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
     wp, sp = mmamd.get_weight(csr, plain.match)

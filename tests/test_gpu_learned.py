@@ -23,11 +23,24 @@ def say(*a):
     sys.stdout.flush()
 
 
+def shuffled_rows(csr, seed):
+    """Rows in random order: the CSR no longer has ascending rows, so the
+    rounds take the exact respond scan (PULL) instead of proposal atomics."""
+    rng = np.random.default_rng(seed)
+    nb, w = csr.neighbours.copy(), csr.weights.copy()
+    for i in range(csr.n):
+        a, b = csr.ranges[i]
+        p = rng.permutation(b - a)
+        nb[a:b], w[a:b] = nb[a:b][p], w[a:b][p]
+    return mmamd.CSR(csr.ranges.copy(), nb, w)
+
+
 def graphs():
     return [("rgg13", mmamd.gen_rgg(1 << 13, seed=31, weighted=True)),
             ("rmat12", mmamd.gen_rmat(12, seed=32, weighted=True)),
             ("grid", mmamd.gen_grid(70, 50, seed=33, weighted=True)),
-            ("rgg12u", mmamd.gen_rgg(1 << 12, seed=34, weighted=False))]
+            ("rgg12u", mmamd.gen_rgg(1 << 12, seed=34, weighted=False)),
+            ("rgg12_shuffled", shuffled_rows(mmamd.gen_rgg(1 << 12, seed=35, weighted=True), 36))]
 
 
 @pytest.mark.parametrize("keep", [False, True])
