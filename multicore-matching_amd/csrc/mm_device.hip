@@ -975,36 +975,58 @@ int32_t analyse_csr(mm_graph *g, const int32_t *ranges, const int32_t *nbrs, con
     const int32_t n = g->n;
     const int64_t nnz = g->nnz;
     std::vector<int32_t> bad(64, 0), unsorted(64, 0), asym(64, 0), wasym(64, 0), maxdeg(64, 0);
+    std::vector<int64_t> fwd(64, 0), bwd(64, 0);
     std::vector<std::vector<int64_t>> hist(64, std::vector<int64_t>(33, 0));
-    mm::parallel_for(n, [&](int64_t b, int64_t e, unsigned t) {
+    constexpr int64_t GRAIN = 1024;  // rows per work item; power-law rows are skewed
+    mm::parallel_for_dynamic(n, GRAIN, [&](int64_t b, int64_t e, unsigned t) {
         for (int64_t i = b; i < e; ++i) {
             const int32_t lo = ranges[2 * i], hi = ranges[2 * i + 1];
             if (lo < 0 || hi < lo || (int64_t)hi > nnz) { bad[t] = 1; continue; }
             const int32_t deg = hi - lo;
             maxdeg[t] = std::max(maxdeg[t], deg);
             if (deg > 0) ++hist[t][31 - __builtin_clz((unsigned)deg)];
-            for (int32_t j = lo; j < hi; ++j)
+            for (int32_t j = lo; j < hi; ++j) {
                 if (nbrs[j] < 0 || nbrs[j] >= n) bad[t] = 1;
+                if (j > lo && nbrs[j] <= nbrs[j - 1]) unsorted[t] = 1;
+            }
         }
     });
     for (int t = 0; t < 64; ++t)
         if (bad[t]) return mm::set_error(MM_ERR_INVALID, "malformed CSR: range or neighbour id out of bounds");
-    mm::parallel_for(n, [&](int64_t b, int64_t e, unsigned t) {
-        for (int64_t i = b; i < e; ++i) {
-            const int32_t lo = ranges[2 * i], hi = ranges[2 * i + 1];
-            for (int32_t j = lo; j < hi; ++j) {
-                if (j > lo && nbrs[j] <= nbrs[j - 1]) unsorted[t] = 1;
-                const int32_t u = nbrs[j];
-                const int32_t *rb = nbrs + ranges[2 * u], *re = nbrs + ranges[2 * u + 1];
-                const int32_t *it = std::lower_bound(rb, re, (int32_t)i);
-                if (it == re || *it != (int32_t)i) { asym[t] = 1; continue; }
-                if (w) {
-                    const float a = w[j], c = w[it - nbrs];
-                    if (!(a == c || (a != a && c != c))) wasym[t] = 1;
+    bool sorted = true;
+    for (int t = 0; t < 64; ++t) sorted = sorted && !unsorted[t];
+    // Symmetry (and, for weights, w(u,v) == w(v,u)) with strictly ascending
+    // rows: every forward entry (i, u), i < u, must find i in u's row; the
+    // reverses found are distinct backward entries, so equal forward and
+    // backward counts make that a bijection.
+    if (sorted)
+        mm::parallel_for_dynamic(n, GRAIN, [&](int64_t b, int64_t e, unsigned t) {
+            for (int64_t i = b; i < e; ++i) {
+                const int32_t lo = ranges[2 * i], hi = ranges[2 * i + 1];
+                for (int32_t j = lo; j < hi; ++j) {
+                    const int32_t u = nbrs[j];
+                    if (u < (int32_t)i) {
+                        ++bwd[t];
+                        continue;
+                    }
+                    if (u == (int32_t)i) continue;
+                    ++fwd[t];
+                    const int32_t *rb = nbrs + ranges[2 * u], *re = nbrs + ranges[2 * u + 1];
+                    const int32_t *it = std::lower_bound(rb, re, (int32_t)i);
+                    if (it == re || *it != (int32_t)i) { asym[t] = 1; continue; }
+                    if (w) {
+                        const float a = w[j], c = w[it - nbrs];
+                        if (!(a == c || (a != a && c != c))) wasym[t] = 1;
+                    }
                 }
             }
-        }
-    });
+        });
+    int64_t nf = 0, nbk = 0;
+    for (int t = 0; t < 64; ++t) {
+        nf += fwd[t];
+        nbk += bwd[t];
+    }
+    if (nf != nbk) asym[0] = 1;
     int us = 0, as = 0, ws = 0;
     for (int t = 0; t < 64; ++t) {
         us |= unsorted[t];
@@ -1141,7 +1163,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     std::vector<int32_t> h_nbrs((size_t)nnz);
     std::vector<int2> h_adj;
     if (g->weighted) h_adj.resize((size_t)nnz);
-    mm::parallel_for(n, [&](int64_t b, int64_t e, unsigned) {
+    mm::parallel_for_dynamic(n, 1024, [&](int64_t b, int64_t e, unsigned) {
         std::vector<int32_t> perm;
         for (int64_t i = b; i < e; ++i) {
             const int32_t o = old_of(i);

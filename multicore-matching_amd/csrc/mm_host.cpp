@@ -51,6 +51,29 @@ void clear_error() { g_last_error.clear(); }
 
 void set_host_threads(unsigned n) { g_host_threads.store(n); }
 
+void parallel_for_dynamic(int64_t total, int64_t grain,
+                          const std::function<void(int64_t, int64_t, unsigned)> &body)
+{
+    const unsigned nt = n_workers();
+    if (total <= 0) return;
+    if (grain < 1) grain = 1;
+    if (nt == 1 || total <= grain) {
+        body(0, total, 0);
+        return;
+    }
+    std::atomic<int64_t> next(0);
+    auto worker = [&](unsigned t) {
+        for (;;) {
+            const int64_t b = next.fetch_add(grain);
+            if (b >= total) break;
+            body(b, std::min(total, b + grain), t);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < nt; ++t) pool.emplace_back(worker, t);
+    for (auto &th : pool) th.join();
+}
+
 void parallel_for(int64_t total, const std::function<void(int64_t, int64_t, unsigned)> &body)
 {
     const unsigned nt = n_workers();

@@ -245,3 +245,31 @@ def test_consecutive_runs_alternate_counter_sets_without_leaking_state():
         np.testing.assert_array_equal(res.match, exp)
         np.testing.assert_array_equal(res.active, act)
         np.testing.assert_array_equal(res.matched, mat)
+
+
+def test_push_eligibility_of_crafted_csrs():
+    """PUSH needs strictly ascending rows, a symmetric structure and symmetric
+    weights; the check pairs every forward entry with its reverse and compares
+    forward and backward counts."""
+    def csr(rows, weights=None):
+        offs = np.cumsum([0] + [len(r) for r in rows])
+        nb = np.array([u for r in rows for u in r], np.int32)
+        w = None if weights is None else np.array([x for r in weights for x in r], np.float32)
+        return CSR.from_offsets(offs, nb, w)
+
+    cases = [
+        (csr([[1, 2], [0, 2], [0, 1]]), True),                                   # triangle
+        (csr([[1], [2], [0]]), False),                                            # directed cycle
+        (csr([[1], [], [0]]), False),                                             # counts equal, reverse missing
+        (csr([[0, 1], [0, 1]]), True),                                            # self-loops ignored
+        (csr([[2, 1], [0], [0]]), False),                                         # unsorted row
+        (csr([[1, 2], [0], [0]], [[0.5, 0.25], [0.5], [0.75]]), False),           # asymmetric weight
+        (csr([[1, 2], [0], [0]], [[0.5, 0.25], [0.5], [0.25]]), True),
+    ]
+    for k, (c, ok) in enumerate(cases):
+        g = Graph(c, keep_order=True)
+        assert bool(g.info()["push_ok"]) == ok, k
+        for respond in (mmamd.MM_RESPOND_AUTO, MM_RESPOND_PULL):
+            res = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS, respond=respond)
+            exp = oracle.match(c, True, True, SEEDS)[0]
+            np.testing.assert_array_equal(res.match, exp, err_msg=str(k))
