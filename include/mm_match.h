@@ -173,6 +173,33 @@ typedef struct mm_tune_result {
 int32_t mm_autotune(mm_graph *g, const mm_opts *opts, const mm_tune_opts *t, mm_tune_result *out);
 int32_t mm_batch_autotune(mm_batch *b, const mm_opts *opts, const mm_tune_opts *t, mm_tune_result *out);
 
+/* Result metrics on the device (src/matchcpu.cu:64-97 getWeight, 127-246
+ * testMatching) of the handle's current result, in graph-local values for a
+ * batch: size = matched vertices, weight summed in double over both endpoints
+ * and halved, and the testMatching counts.  Groups of equal match values are
+ * formed per graph; a group's "first two" members are its two smallest
+ * internal ids (the reference leaves that order to an unstable sort), so for
+ * arrays with groups of three or more, `invalid` may differ from the host's.
+ * Match values >= 4 + n are counted as matched to themselves.  The weight of a
+ * matched pair is exact for CSRs without repeated neighbours. */
+typedef struct mm_check {
+    int64_t under;      /* matched to themselves */
+    int64_t multi;      /* matched to more than one vertex */
+    int64_t invalid;    /* not matched to a neighbour */
+    int64_t matchable;  /* unmatched vertices with an unmatched neighbour */
+    int64_t size;       /* matched vertices (getWeight's size) */
+    double weight;      /* getWeight's weight */
+    int32_t valid;      /* under == multi == invalid == 0 */
+} mm_check;
+int32_t mm_graph_check(mm_graph *g, mm_check *out);
+/* Replaces the handle's current result with match (host, n entries, caller's
+ * order) -- to check an array computed elsewhere. */
+int32_t mm_graph_set_match(mm_graph *g, const int32_t *match);
+/* Batch: totals, and per graph (arrays of n_graphs, each may be NULL) the
+ * weight, the matched vertices and whether the graph's result is valid. */
+int32_t mm_batch_check(mm_batch *b, mm_check *total, double *graph_weight, int64_t *graph_size,
+                       int32_t *graph_valid);
+
 /* Result metrics of a match array (host; src/matchcpu.cu:64-97 getWeight and
  * 127-246 testMatching).  mm_get_weight: size = matched VERTICES, weight summed
  * in double over both endpoints and halved.  mm_test_matching fills

@@ -953,7 +953,7 @@ void release(mm_graph *g)
     void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_match, g->d_req,   g->d_st,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
-                    g->d_unperm, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_params,
+                    g->d_unperm, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_params, g->d_vbase, g->d_chk, g->d_cpart,
                     g->d_lsamp};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -1152,8 +1152,11 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     auto old_of = [&](int64_t i) -> int32_t { return g->relabelled ? order[(size_t)i] : (int32_t)i; };
     std::vector<int32_t> h_ranges(2 * nn), h_loc(nn);
     int64_t off = 0;
+    std::vector<int32_t> h_vbase;
+    if (vbase_old) h_vbase.resize(nn);
     for (int32_t i = 0; i < n; ++i) {
         const int32_t o = old_of(i);
+        if (vbase_old) h_vbase[(size_t)i] = vbase_old[(size_t)o];
         const int32_t deg = ranges[2 * (size_t)o + 1] - ranges[2 * (size_t)o];
         h_ranges[2 * (size_t)i] = (int32_t)off;
         h_ranges[2 * (size_t)i + 1] = (int32_t)(off + deg);
@@ -1199,6 +1202,11 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         if (hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
             (g->weighted &&
              hipMemcpy(g->d_adj, h_adj.data(), sizeof(int2) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess))
+            return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+    }
+    if (vbase_old && n > 0) {
+        if ((rc = dev_alloc(g, &g->d_vbase, nn, "graph bases")) != MM_OK) return fail(rc);
+        if (hipMemcpy(g->d_vbase, h_vbase.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess)
             return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
     }
     if (g->relabelled) {

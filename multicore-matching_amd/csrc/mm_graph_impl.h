@@ -56,6 +56,9 @@ struct mm_graph {
     int2 *d_unperm = nullptr;        // {original position, internal id} pairs, grouped by eighth of internal ids
     int32_t unperm_off[9] = {0};     // group offsets in d_unperm
     int32_t *d_match_out = nullptr;  // result in the caller's numbering, when relabelled
+    int32_t *d_vbase = nullptr;      // batches: first global id of each internal vertex's graph
+    int32_t *d_chk = nullptr;        // [3][n] scratch of the device result check
+    unsigned long long *d_cpart = nullptr;  // per-block partial sums of the device result check
     bool relabelled = false;
     int32_t *d_voff = nullptr;
     uint8_t *d_vround = nullptr;

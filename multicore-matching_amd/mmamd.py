@@ -160,6 +160,16 @@ def _tune_dict(r: _TuneResult) -> dict:
     return {k: getattr(r, k) for k, _ in _TuneResult._fields_}
 
 
+class _Check(ctypes.Structure):
+    _fields_ = [("under", ctypes.c_int64), ("multi", ctypes.c_int64), ("invalid", ctypes.c_int64),
+                ("matchable", ctypes.c_int64), ("size", ctypes.c_int64), ("weight", ctypes.c_double),
+                ("valid", ctypes.c_int32)]
+
+
+def _check_dict(c: _Check) -> dict:
+    return {k: getattr(c, k) for k, _ in _Check._fields_}
+
+
 # Every symbol include/*.h declares, with its ctypes signature.
 _P = ctypes.c_void_p
 _I32, _I64, _U64, _D = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
@@ -181,6 +191,9 @@ SYMBOLS = {
     "mm_csr_free": (None, [ctypes.POINTER(_Csr)]),
     "mm_get_weight": (_I32, [_I32, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
     "mm_test_matching": (_I32, [_I32, _P, _P, _P, _P]),
+    "mm_graph_check": (_I32, [_P, ctypes.POINTER(_Check)]),
+    "mm_graph_set_match": (_I32, [_P, _P]),
+    "mm_batch_check": (_I32, [_P, ctypes.POINTER(_Check), _P, _P, _P]),
     "mm_autotune": (_I32, [_P, ctypes.POINTER(_Opts), ctypes.POINTER(_TuneOpts), ctypes.POINTER(_TuneResult)]),
     "mm_batch_autotune": (_I32, [_P, ctypes.POINTER(_Opts), ctypes.POINTER(_TuneOpts),
                                  ctypes.POINTER(_TuneResult)]),
@@ -457,6 +470,19 @@ class Graph:
         del keep
         return _tune_dict(r)
 
+    def check(self) -> dict:
+        """getWeight / testMatching of the current device result, on the device (mm_graph_check)."""
+        c = _Check()
+        _check(lib().mm_graph_check(self._h, ctypes.byref(c)))
+        return _check_dict(c)
+
+    def set_match(self, match):
+        """Replaces the device result with a host array in the caller's order (mm_graph_set_match)."""
+        m = np.ascontiguousarray(match, np.int32)
+        if m.shape != (self.n,):
+            raise ValueError("match must hold one value per vertex")
+        _check(lib().mm_graph_set_match(self._h, _ptr(m)))
+
     def match_type(self, type_code: int, seeds=None, **kw) -> MatchResult:
         kind, maximal = TYPE_TO_KIND[type_code]
         return self.match(kind=kind, maximal=bool(maximal), seeds=seeds, **kw)
@@ -531,6 +557,14 @@ class Batch:
                                        ctypes.byref(r)))
         del keep
         return _tune_dict(r)
+
+    def check(self):
+        """(totals, per-graph weight, per-graph matched vertices, per-graph valid) on the device."""
+        c = _Check()
+        k = len(self.graphs)
+        gw, gs, gv = np.zeros(k, np.float64), np.zeros(k, np.int64), np.zeros(k, np.int32)
+        _check(lib().mm_batch_check(self._h, ctypes.byref(c), _ptr(gw), _ptr(gs), _ptr(gv)))
+        return _check_dict(c), gw, gs, gv.astype(bool)
 
     def close(self):
         if getattr(self, "_h", None):
