@@ -75,6 +75,9 @@ struct mm_graph {
     float *d_lfeat = nullptr;       // [n][8] features of the current round (traces only)
     int32_t *d_lctr = nullptr;      // [MM_MAX_ROUNDS + 2][4] per-round counters of the learned rounds
     int32_t *d_lpart = nullptr;     // per-block partial counts of the current learned round
+    int32_t *d_lcnt = nullptr;      // per-block work-list lengths of the current learned round
+    int32_t *d_llist = nullptr;     // work lists of the learned rounds (active, blue)
+    size_t llist_cap = 0;
     float *d_params = nullptr;      // network parameters
     unsigned long long *d_lsamp = nullptr;  // [2] samples written / offered
     int32_t *h_lctr = nullptr;      // pinned

@@ -42,7 +42,10 @@ constexpr int LCTR = 4;  // per round: active at select, blue, newly matched, ne
 constexpr int LPART = 4096;  // most blocks of a counting launch
 constexpr int LSLOTS = MM_MAX_ROUNDS + 2;
 constexpr int32_t LREQ_NONE = -1, LREQ_DEAD = -2;
-constexpr uint8_t LC_BLUE = 0, LC_RED = 1, LC_RETIRE = 2;
+// col[v]: this round's colour of an active vertex, LC_DONE once it left the rounds.
+constexpr uint8_t LC_BLUE = 0, LC_RED = 1, LC_RETIRE = 2, LC_DONE = 3;
+// st[v]: state after the previous round: 0 blue, 1 red, 2 dead, 3 matched
+// (match[v] in one byte, the form neighbour gathers read).
 constexpr int SEL_G = 8;   // lanes per row in the select's feature scans
 constexpr int SCAN_G = 4;  // lanes per row in the request / respond scans
 
@@ -73,6 +76,39 @@ __host__ __device__ inline float mlp_z(const float *__restrict__ P, const float 
         z = fmaf(P[MM_MLP_OFF_W3 + j], relu(a), z);
     }
     return z;
+}
+
+// mlp_z for two feature rows at once: every weight is loaded once for both,
+// and each row's arithmetic is exactly mlp_z's.
+__device__ inline void mlp_z2(const float *__restrict__ P, const float *x0, const float *x1, float &z0, float &z1)
+{
+    float h0[NH], h1[NH];
+#pragma unroll
+    for (int j = 0; j < NH; ++j) {
+        float a0 = P[MM_MLP_OFF_B1 + j], a1 = a0;
+#pragma unroll
+        for (int k = 0; k < NF; ++k) {
+            const float wjk = P[MM_MLP_OFF_W1 + j * NF + k];
+            a0 = fmaf(wjk, x0[k], a0);
+            a1 = fmaf(wjk, x1[k], a1);
+        }
+        h0[j] = relu(a0);
+        h1[j] = relu(a1);
+    }
+    z0 = P[MM_MLP_OFF_B3];
+    z1 = z0;
+#pragma unroll 2
+    for (int j = 0; j < NH; ++j) {
+        float a0 = P[MM_MLP_OFF_B2 + j], a1 = a0;
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const float wjk = P[MM_MLP_OFF_W2 + j * NH + k];
+            a0 = fmaf(wjk, h0[k], a0);
+            a1 = fmaf(wjk, h1[k], a1);
+        }
+        z0 = fmaf(P[MM_MLP_OFF_W3 + j], relu(a0), z0);
+        z1 = fmaf(P[MM_MLP_OFF_W3 + j], relu(a1), z1);
+    }
 }
 
 // floor(65536 * s(z)), s(z) = 1 / (1 + 2^(-z log2 e)); 2^f by the degree-7
@@ -119,6 +155,7 @@ struct LDev {
     int32_t *match;
     int32_t *req;
     uint8_t *col;
+    uint8_t *st;
     float *reqw;
     float *feat;
     int32_t *ctr;  // [LSLOTS][LCTR]
@@ -127,6 +164,9 @@ struct LDev {
     int32_t *sv, *sr;
     unsigned long long *scount;
     unsigned long long *resp;  // PUSH proposal keys, one per vertex
+    int32_t *lact, *lblue;     // work lists of the round: block b owns [b * seg, (b + 1) * seg)
+    int32_t *lcnt;             // [2][LPART] per-block list lengths (active, blue)
+    int64_t seg;
 };
 
 // Row entry j in matching order: weight-sorted pairs (id complemented when the
@@ -163,13 +203,21 @@ __device__ __forceinline__ int block_sum(int x)
     return s;
 }
 
+__global__ void k_l_start(int32_t *match, uint8_t *st, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        match[i] = 0;
+        st[i] = 0;
+    }
+}
+
 __global__ void k_l_fill(int32_t *p, int64_t count, int32_t value)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = value;
 }
 
-constexpr int LTILE = 256;  // vertices per select tile = threads per select block
+constexpr int LTILE = 256;  // vertices per tile = threads per block of every round kernel
 
 // Sum over the G aligned lanes of a group (G a power of two <= 64).
 template <int G>
@@ -193,31 +241,145 @@ __device__ __forceinline__ void group_min2(int &p1, int &p2)
     }
 }
 
-// Select of round r over tiles of LTILE vertices.  Phase 1: groups of G lanes
-// scan the rows of the tile's active vertices (coalesced chunks of G entries)
-// for the unmatched / previously-red neighbour counts and the positions of the
-// first two candidates in row order, into LDS.  Phase 2: one thread per vertex
-// reads those two entries, hashes, evaluates the network and draws the
-// colour; PUSH rounds also clear the vertex's proposal key here, before the
-// request phase posts to it.  The network parameters arrive as their own
-// read-only, non-aliased argument, so their wave-uniform loads are scalar.
+// Exclusive prefix of flag over the LTILE threads of the block: this thread's
+// position among the flagged ones; total receives their number.  Every thread
+// must call it.
+__device__ __forceinline__ int block_prefix(bool flag, int &total)
+{
+    __shared__ int wsum[LTILE / 64];
+    const unsigned long long m = __ballot(flag);
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) wsum[wid] = (int)__popcll(m);
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int k = 0; k < LTILE / 64; ++k) {
+        const int c = wsum[k];
+        if (k < wid) before += c;
+        total += c;
+    }
+    return before + (int)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Select of round r.  Block b takes tiles b, b + gridDim, ... of STILE
+// vertices and keeps the active ones of each tile in LDS, packed.  Phase 1:
+// groups of G lanes scan the packed vertices' rows (coalesced chunks of G
+// entries) for the unmatched / previously-red neighbour counts and the row
+// positions of the first two candidates.  Phase 2: each thread takes two
+// packed vertices, reads those entries, hashes, evaluates the network for both
+// at once and draws the colours (PUSH rounds also clear the proposal keys).
+// The active and the blue vertices go to block b's segments of the round's
+// work lists, so the other phases touch only them.  The network parameters
+// arrive as their own read-only, non-aliased argument, so their wave-uniform
+// loads are scalar.
+constexpr int STILE = 2 * LTILE;  // vertices per select tile
+
+struct SelOut {
+    int c;
+    float x[NF];
+    bool need_net;
+};
+
+// Features and plain colour of packed vertex k of the tile (phase 2).
+template <bool A8>
+__device__ __forceinline__ void sel_features(const LDev &d, int r, uint32_t seed, bool scan, int32_t v,
+                                             const int32_t *s_p1, const int32_t *s_p2, const int32_t *s_alive,
+                                             const int32_t *s_redp, int k, uint32_t &vl, SelOut &o)
+{
+    vl = (uint32_t)d.loc[v];
+    const uint32_t hv = mm_select_hash(vl, seed);
+    o.c = hv < d.barrier ? LC_BLUE : LC_RED;
+    o.need_net = false;
+    if (!scan) return;
+    const int2 rg = d.ranges[v];
+    const int32_t p1 = s_p1[k], p2 = s_p2[k], alive = s_alive[k];
+    float w1 = 0.0f, w2 = 0.0f, bit = 0.0f;
+    if (p1 != INT_MAX) {
+        int32_t u;
+        bool valid;
+        l_entry<A8>(d, p1, u, valid, w1);
+        bit = hv < mm_select_hash((uint32_t)d.loc[u], seed) ? 1.0f : 0.0f;
+    }
+    if (p2 != INT_MAX) {
+        int32_t u;
+        bool valid;
+        l_entry<A8>(d, p2, u, valid, w2);
+    }
+    o.x[0] = (float)min(rg.y - rg.x, 1024) * 0.015625f;
+    o.x[1] = w1;
+    o.x[2] = w2;
+    o.x[3] = w1 > 0.0f ? (float)((double)w2 / (double)w1) : 0.0f;
+    o.x[4] = (float)min(alive, 1024) * 0.015625f;
+    o.x[5] = (float)min(s_redp[k], 1024) * 0.015625f;
+    o.x[6] = (float)(r - 1) * 0.03125f;
+    o.x[7] = bit;
+    if (d.trace) {
+#pragma unroll
+        for (int q = 0; q < NF; ++q) d.feat[(size_t)v * NF + q] = o.x[q];
+    }
+    if (d.use_mlp && r <= MM_LEARN_ROUNDS) {
+        if (alive == 0) o.c = LC_RETIRE;
+        else o.need_net = true;
+    }
+}
+
+__device__ __forceinline__ int sel_draw(uint32_t vl, uint32_t seed, float z, int plain)
+{
+    if (!isfinite(z)) return plain;
+    const uint32_t T = blue_threshold(z);
+    return (mm_select_hash(vl, seed ^ MM_LEARN_DRAW_TWEAK) >> 16) < T ? LC_BLUE : LC_RED;
+}
+
 template <bool A8, bool PUSH, int G>
 __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed, const float *__restrict__ params)
 {
-    __shared__ int32_t s_p1[LTILE], s_p2[LTILE], s_alive[LTILE], s_redp[LTILE];
+    __shared__ int32_t s_idx[STILE], s_p1[STILE], s_p2[STILE], s_alive[STILE], s_redp[STILE];
     const int t = threadIdx.x;
     const int gl = t & (G - 1);
     const int grp = t / G;
     constexpr int GROUPS = LTILE / G;
-    const bool net = d.use_mlp && r <= MM_LEARN_ROUNDS;
-    const bool scan = net || d.trace;
+    const bool scan = (d.use_mlp && r <= MM_LEARN_ROUNDS) || d.trace;
+    int32_t *act_seg = d.lact + (size_t)blockIdx.x * d.seg;
+    int32_t *blue_seg = d.lblue + (size_t)blockIdx.x * d.seg;
     int n_act = 0, n_blue = 0;
-    for (int64_t base = (int64_t)blockIdx.x * LTILE; base < d.n; base += (int64_t)gridDim.x * LTILE) {
+    for (int64_t base = (int64_t)blockIdx.x * STILE; base < d.n; base += (int64_t)gridDim.x * STILE) {
+        int A0, A1;
+        const bool act0 = base + t < d.n && d.match[base + t] < 2;
+        const bool act1 = base + LTILE + t < d.n && d.match[base + LTILE + t] < 2;
+        const int pos0 = block_prefix(act0, A0);
+        if (act0) s_idx[pos0] = t;
+        const int pos1 = block_prefix(act1, A1);
+        if (act1) s_idx[A0 + pos1] = LTILE + t;
+        const int A = A0 + A1;
+        __syncthreads();
         if (scan) {
-            for (int k = grp; k < LTILE; k += GROUPS) {
-                const int64_t v = base + k;
-                const int32_t m = v < d.n ? d.match[v] : 4;
-                if (m >= 2) continue;
+            if (r == 1) {
+                // Round 1: every vertex is unmatched, so a row's unmatched
+                // neighbours are all of it, none was red before, and its first
+                // two candidates are its first two entries that can win (rows
+                // keep those first).
+                for (int k = t; k < A; k += LTILE) {
+                    const int2 rg = d.ranges[base + s_idx[k]];
+                    int32_t p1 = INT_MAX, p2 = INT_MAX;
+                    for (int32_t j = rg.x; j < rg.y && j < rg.x + 2; ++j) {
+                        int32_t u;
+                        bool valid;
+                        float w;
+                        l_entry<A8>(d, j, u, valid, w);
+                        if (!valid) break;
+                        if (p1 == INT_MAX) p1 = j;
+                        else p2 = j;
+                    }
+                    s_p1[k] = p1;
+                    s_p2[k] = p2;
+                    s_alive[k] = rg.y - rg.x;
+                    s_redp[k] = 0;
+                }
+            } else
+            for (int k = grp; k < A; k += GROUPS) {
+                const int64_t v = base + s_idx[k];
                 const int2 rg = d.ranges[v];
                 int alive = 0, redp = 0, p1 = INT_MAX, p2 = INT_MAX;
                 for (int32_t j = rg.x + gl; j < rg.y; j += G) {
@@ -225,10 +387,10 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
                     bool valid;
                     float w;
                     l_entry<A8>(d, j, u, valid, w);
-                    const int32_t mu = d.match[u];
-                    if (mu >= 4) continue;
+                    const int su = d.st[u];
+                    if (su >= 3) continue;
                     ++alive;
-                    redp += mu == 1;
+                    redp += su == 1;
                     if (valid) {
                         if (p1 == INT_MAX) p1 = j;
                         else if (p2 == INT_MAX) p2 = j;
@@ -246,67 +408,53 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
             }
             __syncthreads();
         }
-        const int64_t v = base + t;
-        const int32_t m = v < d.n ? d.match[v] : 4;
-        const bool act = m < 2;
-        int c = LC_RED;
-        if (act) {
-            const uint32_t vl = (uint32_t)d.loc[v];
-            const uint32_t hv = mm_select_hash(vl, seed);
-            const int plain = hv < d.barrier ? LC_BLUE : LC_RED;
-            c = plain;
-            if (scan) {
-                const int2 rg = d.ranges[v];
-                const int32_t p1 = s_p1[t], p2 = s_p2[t], alive = s_alive[t];
-                float w1 = 0.0f, w2 = 0.0f, bit = 0.0f;
-                if (p1 != INT_MAX) {
-                    int32_t u;
-                    bool valid;
-                    l_entry<A8>(d, p1, u, valid, w1);
-                    bit = hv < mm_select_hash((uint32_t)d.loc[u], seed) ? 1.0f : 0.0f;
-                }
-                if (p2 != INT_MAX) {
-                    int32_t u;
-                    bool valid;
-                    l_entry<A8>(d, p2, u, valid, w2);
-                }
-                float x[NF];
-                x[0] = (float)min(rg.y - rg.x, 1024) * 0.015625f;
-                x[1] = w1;
-                x[2] = w2;
-                x[3] = w1 > 0.0f ? (float)((double)w2 / (double)w1) : 0.0f;
-                x[4] = (float)min(alive, 1024) * 0.015625f;
-                x[5] = (float)min(s_redp[t], 1024) * 0.015625f;
-                x[6] = (float)(r - 1) * 0.03125f;
-                x[7] = bit;
-                if (d.trace) {
-#pragma unroll
-                    for (int k = 0; k < NF; ++k) d.feat[(size_t)v * NF + k] = x[k];
-                }
-                if (net) {
-                    if (alive == 0) {
-                        c = LC_RETIRE;
-                    } else {
-                        const float z = mlp_z(params, x);
-                        if (isfinite(z)) {
-                            const uint32_t T = blue_threshold(z);
-                            c = (mm_select_hash(vl, seed ^ MM_LEARN_DRAW_TWEAK) >> 16) < T ? LC_BLUE : LC_RED;
-                        }
-                    }
-                }
-            }
-            d.col[v] = (uint8_t)c;
-            if (PUSH) d.resp[v] = 0ull;
+        // phase 2: packed vertices t and t + LTILE
+        const bool has0 = t < A, has1 = t + LTILE < A;
+        int32_t v0 = -1, v1 = -1;
+        uint32_t vl0 = 0, vl1 = 0;
+        SelOut o0, o1;
+        o0.c = o1.c = LC_RED;
+        o0.need_net = o1.need_net = false;
+        if (has0) {
+            v0 = (int32_t)(base + s_idx[t]);
+            sel_features<A8>(d, r, seed, scan, v0, s_p1, s_p2, s_alive, s_redp, t, vl0, o0);
         }
-        n_act += act;
-        n_blue += act && c == LC_BLUE;
-        if (scan) __syncthreads();
+        if (has1) {
+            v1 = (int32_t)(base + s_idx[t + LTILE]);
+            sel_features<A8>(d, r, seed, scan, v1, s_p1, s_p2, s_alive, s_redp, t + LTILE, vl1, o1);
+        }
+        if (o0.need_net && o1.need_net) {
+            float z0, z1;
+            mlp_z2(params, o0.x, o1.x, z0, z1);
+            o0.c = sel_draw(vl0, seed, z0, o0.c);
+            o1.c = sel_draw(vl1, seed, z1, o1.c);
+        } else if (o0.need_net) {
+            o0.c = sel_draw(vl0, seed, mlp_z(params, o0.x), o0.c);
+        } else if (o1.need_net) {
+            o1.c = sel_draw(vl1, seed, mlp_z(params, o1.x), o1.c);
+        }
+        if (has0) {
+            d.col[v0] = (uint8_t)o0.c;
+            if (PUSH) d.resp[v0] = 0ull;
+            act_seg[n_act + t] = v0;
+        }
+        if (has1) {
+            d.col[v1] = (uint8_t)o1.c;
+            if (PUSH) d.resp[v1] = 0ull;
+            act_seg[n_act + LTILE + t] = v1;
+        }
+        int B0, B1;
+        const bool blue0 = has0 && o0.c == LC_BLUE, blue1 = has1 && o1.c == LC_BLUE;
+        const int bpos0 = block_prefix(blue0, B0);
+        if (blue0) blue_seg[n_blue + bpos0] = v0;
+        const int bpos1 = block_prefix(blue1, B1);
+        if (blue1) blue_seg[n_blue + B0 + bpos1] = v1;
+        n_act += A;
+        n_blue += B0 + B1;
     }
-    n_act = block_sum(n_act);
-    n_blue = block_sum(n_blue);
-    if (threadIdx.x == 0) {
-        d.part[0 * LPART + blockIdx.x] = n_act;
-        d.part[1 * LPART + blockIdx.x] = n_blue;
+    if (t == 0) {
+        d.lcnt[blockIdx.x] = n_act;
+        d.lcnt[LPART + blockIdx.x] = n_blue;
     }
 }
 
@@ -335,124 +483,126 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long ball
     return G == 64 ? ballot : (ballot >> shift) & ((1ull << G) - 1ull);
 }
 
-// Request: each blue vertex's group scans its row in matching order, G
-// entries at a time, and stops at the first red unmatched neighbour.
-template <bool A8, bool PUSH, int G>
-__global__ void k_l_request(LDev d)
+// First candidate of v's row in matching order, G lanes per row, G entries
+// per step; RESPOND looks for a blue unmatched proposer of v, else for a red
+// unmatched neighbour (alive: v has an unmatched neighbour).  Every lane of
+// the wavefront must call it, has = false for lanes without a vertex.
+template <bool A8, bool RESPOND, int G>
+__device__ __forceinline__ bool l_first(const LDev &d, bool has, int32_t v, int32_t &best, float &bw, bool &alive)
 {
     const int gl = __lane_id() & (G - 1);
-    const int64_t gstride = (int64_t)gridDim.x * blockDim.x / G;
-    const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const int64_t n_round = ((int64_t)d.n + gstride - 1) / gstride * gstride;
-    for (int64_t v = g0; v < n_round; v += gstride) {
-        bool blue = false;
-        if (v < d.n && d.match[v] < 2) {
-            if (d.col[v] == LC_BLUE) blue = true;
-            else if (gl == 0) d.req[v] = LREQ_NONE;
+    int2 rg = make_int2(0, 0);
+    if (has) rg = d.ranges[v];
+    bool found = false;
+    alive = false;
+    for (int32_t jb = rg.x; __ballot(has && !found && jb < rg.y) != 0ull; jb += G) {
+        const bool live = has && !found && jb < rg.y;
+        const int32_t j = jb + gl;
+        bool cand = false;
+        int32_t u = 0;
+        float w = 0.0f;
+        if (live && j < rg.y) {
+            bool valid;
+            l_entry<A8>(d, j, u, valid, w);
+            // col: this round's colour of an active neighbour, LC_DONE once
+            // it left (an active vertex never neighbours a dead one)
+            const int cu = d.col[u];
+            if (RESPOND) {
+                cand = valid && cu == LC_BLUE && d.req[u] == v;
+            } else {
+                alive |= cu != LC_DONE;
+                cand = valid && cu == LC_RED;
+            }
         }
-        int2 rg = make_int2(0, 0);
-        if (blue) rg = d.ranges[v];
-        bool alive = false, found = false;
+        const unsigned long long gm = group_bits<G>(__ballot(cand));
+        const int src = (__lane_id() & ~(G - 1)) + (gm ? __ffsll((long long)gm) - 1 : 0);
+        const int32_t su = __shfl(u, src);
+        const float sw = __shfl(w, src);
+        if (live && gm) {
+            best = su;
+            bw = sw;
+            found = true;
+        }
+    }
+    if (!RESPOND) alive = group_bits<G>(__ballot(alive)) != 0ull;
+    return found;
+}
+
+// Request: block b's blue list, G lanes per vertex.
+template <bool A8, bool PUSH, int G>
+__global__ __launch_bounds__(LTILE) void k_l_request(LDev d)
+{
+    const int32_t cnt = d.lcnt[LPART + blockIdx.x];
+    const int32_t *list = d.lblue + (size_t)blockIdx.x * d.seg;
+    constexpr int GPB = LTILE / G;
+    const int grp = threadIdx.x / G, gl = threadIdx.x & (G - 1);
+    const int32_t rounded = (cnt + GPB - 1) / GPB * GPB;
+    for (int32_t k = grp; k < rounded; k += GPB) {
+        const bool has = k < cnt;
+        const int32_t v = has ? list[k] : 0;
         int32_t best = -1;
         float bw = 0.0f;
-        for (int32_t jb = rg.x; __ballot(blue && !found && jb < rg.y) != 0ull; jb += G) {
-            const bool live = blue && !found && jb < rg.y;
-            const int32_t j = jb + gl;
-            bool cand = false;
-            int32_t u = 0;
-            float w = 0.0f;
-            if (live && j < rg.y) {
-                bool valid;
-                l_entry<A8>(d, j, u, valid, w);
-                const int32_t mu = d.match[u];
-                alive |= mu < 4;
-                cand = valid && mu < 2 && d.col[u] == LC_RED;
-            }
-            const unsigned long long gm = group_bits<G>(__ballot(cand));
-            const int src = (__lane_id() & ~(G - 1)) + (gm ? __ffsll((long long)gm) - 1 : 0);
-            const int32_t su = __shfl(u, src);
-            const float sw = __shfl(w, src);
-            if (live && gm) {
-                best = su;
-                bw = sw;
-                found = true;
-            }
-        }
-        const bool any_alive = group_bits<G>(__ballot(alive)) != 0ull;
-        if (blue && gl == 0) {
-            d.req[v] = found ? best : (any_alive ? LREQ_NONE : LREQ_DEAD);
+        bool alive;
+        const bool found = l_first<A8, false, G>(d, has, v, best, bw, alive);
+        if (has && gl == 0) {
+            d.req[v] = found ? best : (alive ? LREQ_NONE : LREQ_DEAD);
             d.reqw[v] = found ? bw : 0.0f;
             if (PUSH && found) atomicMax(d.resp + best, l_key(bw, (uint32_t)d.loc[v]));
         }
     }
 }
 
-// Respond: each red vertex's group scans its row in matching order for the
-// first blue unmatched neighbour that proposed to it.
+// Respond (PULL): block b's active list; each red vertex takes its first blue
+// proposer in matching order.
 template <bool A8, int G>
-__global__ void k_l_respond(LDev d)
+__global__ __launch_bounds__(LTILE) void k_l_respond(LDev d)
 {
-    const int gl = __lane_id() & (G - 1);
-    const int64_t gstride = (int64_t)gridDim.x * blockDim.x / G;
-    const int64_t g0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const int64_t n_round = ((int64_t)d.n + gstride - 1) / gstride * gstride;
-    for (int64_t v = g0; v < n_round; v += gstride) {
-        const bool red = v < d.n && d.match[v] < 2 && d.col[v] == LC_RED;
-        int2 rg = make_int2(0, 0);
-        if (red) rg = d.ranges[v];
-        bool found = false;
-        for (int32_t jb = rg.x; __ballot(red && !found && jb < rg.y) != 0ull; jb += G) {
-            const bool live = red && !found && jb < rg.y;
-            const int32_t j = jb + gl;
-            bool cand = false;
-            int32_t u = 0;
-            float w = 0.0f;
-            if (live && j < rg.y) {
-                bool valid;
-                l_entry<A8>(d, j, u, valid, w);
-                cand = valid && d.match[u] < 2 && d.col[u] == LC_BLUE && d.req[u] == (int32_t)v;
-            }
-            const unsigned long long gm = group_bits<G>(__ballot(cand));
-            const int src = (__lane_id() & ~(G - 1)) + (gm ? __ffsll((long long)gm) - 1 : 0);
-            const int32_t su = __shfl(u, src);
-            const float sw = __shfl(w, src);
-            if (live && gm) {
-                if (gl == 0) {
-                    d.req[v] = su;
-                    d.reqw[v] = sw;
-                }
-                found = true;
-            }
+    const int32_t cnt = d.lcnt[blockIdx.x];
+    const int32_t *list = d.lact + (size_t)blockIdx.x * d.seg;
+    constexpr int GPB = LTILE / G;
+    const int grp = threadIdx.x / G, gl = threadIdx.x & (G - 1);
+    const int32_t rounded = (cnt + GPB - 1) / GPB * GPB;
+    for (int32_t k = grp; k < rounded; k += GPB) {
+        const int32_t v = k < cnt ? list[k] : 0;
+        const bool red = k < cnt && d.col[v] == LC_RED;
+        int32_t best = -1;
+        float bw = 0.0f;
+        bool alive;
+        const bool found = l_first<A8, true, G>(d, red, v, best, bw, alive);
+        if (red && gl == 0) {
+            d.req[v] = found ? best : LREQ_NONE;
+            if (found) d.reqw[v] = bw;
         }
     }
 }
 
+// Match: block b's active list, one thread per vertex.
 template <bool PUSH>
-__global__ void k_l_match(LDev d, int r, uint32_t seed)
+__global__ __launch_bounds__(LTILE) void k_l_match(LDev d, int r, uint32_t seed)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int32_t cnt = d.lcnt[blockIdx.x];
+    const int32_t *list = d.lact + (size_t)blockIdx.x * d.seg;
     int n_mat = 0, n_dead = 0;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < d.n; base += stride) {
-        const int32_t v = (int32_t)(base + threadIdx.x);
-        const bool inb = v < d.n;
-        const int32_t m = inb ? d.match[v] : 4;
-        const bool act = m < 2;
-        int32_t nm = m;
+    const int32_t rounded = (cnt + LTILE - 1) / LTILE * LTILE;
+    for (int32_t k = threadIdx.x; k < rounded; k += LTILE) {
+        const bool act = k < cnt;
+        int32_t nm = 0, v = 0;
         int c = LC_RED;
         uint32_t vl = 0;
         if (act) {
+            v = list[k];
             c = d.col[v];
             const int32_t q = d.req[v];
             vl = (uint32_t)d.loc[v];
-            if (c == LC_RETIRE || q == LREQ_DEAD) {
+            if (c == LC_RETIRE || (c == LC_BLUE && q == LREQ_DEAD)) {
                 nm = 2;
             } else if (PUSH) {
                 // red: the winning key names its proposer; blue: it won iff its target's key names it
                 if (c == LC_RED) {
-                    const unsigned long long k = d.resp[v];
-                    if (k != 0ull) {
-                        nm = 4 + (int32_t)min(vl, 0xFFFFFFFFu - (uint32_t)k);
-                        d.reqw[v] = l_key_weight(k);
+                    const unsigned long long key = d.resp[v];
+                    if (key != 0ull) {
+                        nm = 4 + (int32_t)min(vl, 0xFFFFFFFFu - (uint32_t)key);
+                        d.reqw[v] = l_key_weight(key);
                     } else {
                         nm = c;
                     }
@@ -467,6 +617,8 @@ __global__ void k_l_match(LDev d, int r, uint32_t seed)
                 nm = c;
             }
             d.match[v] = nm;
+            d.st[v] = (uint8_t)(nm >= 4 ? 3 : nm);
+            if (nm >= 2) d.col[v] = LC_DONE;
         }
         n_mat += act && nm >= 4;
         n_dead += act && nm == 2;
@@ -482,7 +634,7 @@ __global__ void k_l_match(LDev d, int r, uint32_t seed)
                 const unsigned long long slot = slot0 + __popcll(mask & ((1ull << lane) - 1ull));
                 if (keep && slot < (unsigned long long)d.trace_cap) {
 #pragma unroll
-                    for (int k = 0; k < NF; ++k) d.sx[slot * NF + k] = d.feat[(size_t)v * NF + k];
+                    for (int kk = 0; kk < NF; ++kk) d.sx[slot * NF + kk] = d.feat[(size_t)v * NF + kk];
                     d.sy[slot] = c == LC_BLUE ? 1.0f : 0.0f;
                     d.sw[slot] = d.reqw[v];
                     d.sv[slot] = (int32_t)vl;
@@ -499,13 +651,13 @@ __global__ void k_l_match(LDev d, int r, uint32_t seed)
     }
 }
 
-// ctr[r][c] = sum of the round's per-block partial counts (one block).
-__global__ void k_l_reduce(const int32_t *part, int nsel, int nmatch, int32_t *ctr_r)
+// ctr[r] = {active, blue, newly matched, newly dead} from the round's per-block counts.
+__global__ void k_l_reduce(const int32_t *lcnt, const int32_t *part, int nblk, int32_t *ctr_r)
 {
     const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nb = c < 2 ? nsel : nmatch;
+    const int32_t *src = c < 2 ? lcnt + c * LPART : part + c * LPART;
     int x = 0;
-    for (int b = lane; b < nb; b += 64) x += part[c * LPART + b];
+    for (int b = lane; b < nblk; b += 64) x += src[b];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
     if (lane == 0) ctr_r[c] = x;
@@ -770,6 +922,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         return rc;
     if (trace && (rc = graph_buf(g, &g->d_lfeat, nn * NF, "features")) != MM_OK) return rc;
     if ((rc = graph_buf(g, &g->d_lpart, (size_t)LCTR * LPART, "partial counts")) != MM_OK) return rc;
+    if ((rc = graph_buf(g, &g->d_lcnt, (size_t)2 * LPART, "list lengths")) != MM_OK) return rc;
     if (!g->h_lctr && hipHostMalloc((void **)&g->h_lctr, sizeof(int32_t) * LSLOTS * LCTR) != hipSuccess)
         return mm::set_error(MM_ERR_NOMEM, "pinned host allocation failed");
 
@@ -789,6 +942,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
     d.match = g->d_match;
     d.req = g->d_req;
     d.col = g->d_lcol;
+    d.st = g->d_st;
     d.reqw = g->d_lreqw;
     d.feat = g->d_lfeat;
     d.ctr = g->d_lctr;
@@ -809,12 +963,28 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
 
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>({((int64_t)g->n + block - 1) / block,
                                                                    (int64_t)g->num_cu * (2048 / block), LPART}));
-    const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>({((int64_t)g->n + LTILE - 1) / LTILE,
-                                                                    (int64_t)g->num_cu * 8, LPART}));
+    // Every round kernel runs sgrid blocks; block b's tiles feed its own list segments.
+    const int64_t tiles = ((int64_t)g->n + STILE - 1) / STILE;
+    const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>({tiles, (int64_t)g->num_cu * 8, LPART}));
+    d.seg = (tiles + sgrid - 1) / sgrid * STILE;
+    if ((size_t)(2 * sgrid * d.seg) > g->llist_cap) {
+        if (g->d_llist) (void)hipFree(g->d_llist);
+        g->d_llist = nullptr;
+        g->llist_cap = 0;
+        void *q = nullptr;
+        if ((rc = mm_graph_alloc(g, &q, sizeof(int32_t) * 2 * (size_t)sgrid * (size_t)d.seg, "learned work lists")) !=
+            MM_OK)
+            return rc;
+        g->d_llist = (int32_t *)q;
+        g->llist_cap = (size_t)(2 * sgrid * d.seg);
+    }
+    d.lact = g->d_llist;
+    d.lblue = g->d_llist + (size_t)sgrid * d.seg;
+    d.lcnt = g->d_lcnt;
     ML_TRY(hipEventRecord(g->ev_t1, st), "hipEventRecord");
     hipLaunchKernelGGL(k_l_fill, dim3(1), dim3(256), 0, st, d.ctr, (int64_t)LSLOTS * LCTR, 0);
     hipLaunchKernelGGL(k_l_fill, dim3(1), dim3(64), 0, st, (int32_t *)d.scount, (int64_t)4, 0);
-    if (g->n > 0) hipLaunchKernelGGL(k_l_fill, dim3(grid), dim3(block), 0, st, d.match, (int64_t)g->n, 0);
+    if (g->n > 0) hipLaunchKernelGGL(k_l_start, dim3(grid), dim3(block), 0, st, d.match, d.st, (int64_t)g->n);
     const int R_run = maximal ? std::min(R_max, o->n_seeds) : R_max;
     const int chunk = maximal ? 4 : R_run;
     int rounds = -1;
@@ -826,17 +996,17 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
 #define ML_ROUND(A8_, PUSH_)                                                                                   \
     do {                                                                                                       \
         hipLaunchKernelGGL((k_l_select<A8_, PUSH_, SEL_G>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed, P); \
-        hipLaunchKernelGGL((k_l_request<A8_, PUSH_, SCAN_G>), dim3(grid), dim3(block), 0, st, d);             \
-        if (!PUSH_) hipLaunchKernelGGL((k_l_respond<A8_, SCAN_G>), dim3(grid), dim3(block), 0, st, d);         \
-        hipLaunchKernelGGL((k_l_match<PUSH_>), dim3(grid), dim3(block), 0, st, d, rr, seed);                  \
+        hipLaunchKernelGGL((k_l_request<A8_, PUSH_, SCAN_G>), dim3(sgrid), dim3(LTILE), 0, st, d);            \
+        if (!PUSH_) hipLaunchKernelGGL((k_l_respond<A8_, SCAN_G>), dim3(sgrid), dim3(LTILE), 0, st, d);        \
+        hipLaunchKernelGGL((k_l_match<PUSH_>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed);                 \
     } while (0)
             if (A8 && push) ML_ROUND(true, true);
             else if (A8) ML_ROUND(true, false);
             else if (push) ML_ROUND(false, true);
             else ML_ROUND(false, false);
 #undef ML_ROUND
-            hipLaunchKernelGGL(k_l_reduce, dim3(1), dim3(64 * LCTR), 0, st, (const int32_t *)d.part, sgrid, grid,
-                               d.ctr + (size_t)rr * LCTR);
+            hipLaunchKernelGGL(k_l_reduce, dim3(1), dim3(64 * LCTR), 0, st, (const int32_t *)d.lcnt,
+                               (const int32_t *)d.part, sgrid, d.ctr + (size_t)rr * LCTR);
         }
         ML_TRY(hipGetLastError(), "A HIP error occurred during the matching process");
         if (maximal) {
