@@ -114,7 +114,8 @@ def cpu_matchers(csr, seeds, r99: int) -> dict:
     """The reference's CPU matchers on the same graph, restated in the oracle
     and timed on this host (a reported comparison, never the measured path):
     types 0 (random greedy), 4 (weighted vertex-order greedy) and 5 (edge-sorted
-    greedy, src/matchcpu.cu:259-303, 695-810) single-threaded, and the TBB
+    greedy, src/matchcpu.cu:259-303, 695-810) single-threaded, types 1-3
+    (minimum degree, Karp-Sipser, 304-682) as the library's host matchers, and the TBB
     round matcher restated with OpenMP (type 11 maximal, src/matchtbb.cu:567-623)
     on 1..all cores for the first r99 rounds plus one full maximal run."""
     import oracle
@@ -124,6 +125,14 @@ def cpu_matchers(csr, seeds, r99: int) -> dict:
                      ("type5_edge_sorted_greedy", oracle.cpu_weighted_edge)):
         t0 = time.perf_counter()
         m = fn(csr)
+        dt = time.perf_counter() - t0
+        w, size = mmamd.get_weight(csr, m)
+        out[name] = {"matched": size, "weight": w, "ms": round(dt * 1e3, 2), "cores": 1}
+    # types 1-3 (minimum degree, Karp-Sipser): the library's host matchers
+    for name, t in (("type1_static_min_degree", mmamd.HOST_STAT_MIN_DEG), ("type2_karp_sipser", mmamd.HOST_KARP_SIPSER),
+                    ("type3_dynamic_min_degree", mmamd.HOST_MIN_DEG)):
+        t0 = time.perf_counter()
+        m = mmamd.host_match(csr, t)
         dt = time.perf_counter() - t0
         w, size = mmamd.get_weight(csr, m)
         out[name] = {"matched": size, "weight": w, "ms": round(dt * 1e3, 2), "cores": 1}
@@ -182,7 +191,7 @@ def run_single(args, csr, name: str, workload_desc: str) -> dict:
         tiles = tiles_of(tune)
     else:
         tune = None
-        tiles = {"lanes": args.lanes, "block_size": args.block, "hub_degree": args.hub, "unroll": 0}
+        tiles = {"lanes": args.lanes, "block_size": args.block, "hub_degree": args.hub, "unroll": args.unroll}
     log(f"tiles {tiles} (autotune {tune})")
 
     def step():
@@ -340,6 +349,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--hub", type=int, default=0)
+    ap.add_argument("--unroll", type=int, default=0, help="row entries per thread of the tile kernels (4, 8, 16)")
     ap.add_argument("--respond", type=int, default=mmamd.MM_RESPOND_AUTO)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-selection comparison")

@@ -200,6 +200,15 @@ int32_t mm_graph_set_match(mm_graph *g, const int32_t *match);
 int32_t mm_batch_check(mm_batch *b, mm_check *total, double *graph_weight, int64_t *graph_size,
                        int32_t *graph_valid);
 
+/* Sequential host matchers, types 0-5 of the reference's getMatcher
+ * (bin/match.cpp:136-143; src/matchcpu.cu:259-810): 0 random greedy,
+ * 1 static minimum degree, 2 Karp-Sipser, 3 dynamic minimum degree,
+ * 4 weighted vertex-order greedy, 5 edge-sorted greedy (edges from the rows,
+ * i < j, row order).  Host only; match_out gets n entries in the reference
+ * encoding.  weights may be NULL (unit weights). */
+int32_t mm_host_match(int32_t type, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *neighbours,
+                      const float *weights, int32_t *match_out);
+
 /* Result metrics of a match array (host; src/matchcpu.cu:64-97 getWeight and
  * 127-246 testMatching).  mm_get_weight: size = matched VERTICES, weight summed
  * in double over both endpoints and halved.  mm_test_matching fills

@@ -5,10 +5,10 @@
 //   match [-m "types"] [-t] [-r] [-d device] [-a repetitions] [-b barrier 0..16]
 //         [--gnuplot file] [--scaledata] [--threads "counts"] [--intermediate] graph.{graph,mtx}
 //
-// Types: 0 CPU random, 4 CPU weighted, 5 CPU weighted (edge-based), 6/7 GPU
+// Types: 0 CPU random, 1 CPU static minimum degree, 2 CPU Karp--Sipser, 3 CPU
+// dynamic minimum degree, 4 CPU weighted, 5 CPU weighted (edge-based), 6/7 GPU
 // random (maximal), 8/9 GPU weighted (maximal), 10/11 CPU round matchers on
-// host threads (the reference's TBB types).  Types 1-3 (minimum degree,
-// Karp--Sipser) are not part of this build.  stdout: a header line
+// host threads (the reference's TBB types).  stdout: a header line
 // "<file> <vertices> <edges> <barrier>", then per type "<type> <threads>
 // <size avg dev> <weight avg dev> <total ms avg dev> <match ms avg dev>".
 // --intermediate prints, for GPU types, "round<TAB>weight<TAB>size" after
@@ -67,6 +67,9 @@ GraphMatching *getMatcher(const Graph &graph, int type, int threadsPerBlock, uns
 {
     switch (type) {
     case 0: return new GraphMatchingCPURandom(graph);
+    case 1: return new GraphMatchingCPUStatMinDeg(graph);
+    case 2: return new GraphMatchingCPUKarpSipser(graph);
+    case 3: return new GraphMatchingCPUMinDeg(graph);
     case 4: return new GraphMatchingCPUWeighted(graph);
     case 5: return new GraphMatchingCPUWeightedEdge(graph);
     case 6: return new GraphMatchingGPURandom(graph, threadsPerBlock, barrier);
@@ -75,10 +78,6 @@ GraphMatching *getMatcher(const Graph &graph, int type, int threadsPerBlock, uns
     case 9: return new GraphMatchingGPUWeightedMaximal(graph, threadsPerBlock, barrier);
     case 10: return new GraphMatchingTBBRandom(graph, barrier);
     case 11: return new GraphMatchingTBBWeighted(graph, barrier);
-    case 1:
-    case 2:
-    case 3: cerr << "Matching type " << type << " (minimum degree / Karp-Sipser) is not part of this build!" << endl;
-            throw exception();
     default: cerr << "Unknown matching type!" << endl; throw exception();
     }
 }

@@ -38,13 +38,40 @@ class GraphMatching
 };
 
 // Sequential CPU matchers (src/matchcpu.h:50-108): type 0 random greedy,
-// type 4 weighted vertex-order greedy, type 5 edge-sorted greedy.  The minimum
-// degree and Karp--Sipser matchers (types 1-3) are not part of this build.
+// 1 static minimum degree, 2 Karp--Sipser, 3 dynamic (two-sided) minimum
+// degree, 4 weighted vertex-order greedy, 5 edge-sorted greedy.
 class GraphMatchingCPURandom : public GraphMatching
 {
     public:
         GraphMatchingCPURandom(const Graph &);
         ~GraphMatchingCPURandom();
+
+        void performMatching(std::vector<int> &, hipEvent_t &, hipEvent_t &) const;
+};
+
+class GraphMatchingCPUMinDeg : public GraphMatching
+{
+    public:
+        GraphMatchingCPUMinDeg(const Graph &);
+        ~GraphMatchingCPUMinDeg();
+
+        void performMatching(std::vector<int> &, hipEvent_t &, hipEvent_t &) const;
+};
+
+class GraphMatchingCPUStatMinDeg : public GraphMatching
+{
+    public:
+        GraphMatchingCPUStatMinDeg(const Graph &);
+        ~GraphMatchingCPUStatMinDeg();
+
+        void performMatching(std::vector<int> &, hipEvent_t &, hipEvent_t &) const;
+};
+
+class GraphMatchingCPUKarpSipser : public GraphMatching
+{
+    public:
+        GraphMatchingCPUKarpSipser(const Graph &);
+        ~GraphMatchingCPUKarpSipser();
 
         void performMatching(std::vector<int> &, hipEvent_t &, hipEvent_t &) const;
 };

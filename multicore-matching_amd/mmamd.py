@@ -191,6 +191,7 @@ SYMBOLS = {
     "mm_csr_free": (None, [ctypes.POINTER(_Csr)]),
     "mm_get_weight": (_I32, [_I32, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_I64)]),
     "mm_test_matching": (_I32, [_I32, _P, _P, _P, _P]),
+    "mm_host_match": (_I32, [_I32, _I32, _I64, _P, _P, _P, _P]),
     "mm_graph_check": (_I32, [_P, ctypes.POINTER(_Check)]),
     "mm_graph_set_match": (_I32, [_P, _P]),
     "mm_batch_check": (_I32, [_P, ctypes.POINTER(_Check), _P, _P, _P]),
@@ -302,6 +303,21 @@ def get_weight(csr: CSR, match: np.ndarray) -> tuple[float, int]:
     wt, sz = ctypes.c_double(), ctypes.c_int64()
     _check(lib().mm_get_weight(csr.n, _ptr(r), _ptr(nb), _ptr(w), _ptr(m), ctypes.byref(wt), ctypes.byref(sz)))
     return float(wt.value), int(sz.value)
+
+
+# Sequential host matcher types (bin/match.cpp:136-143).
+HOST_RANDOM, HOST_STAT_MIN_DEG, HOST_KARP_SIPSER, HOST_MIN_DEG, HOST_WEIGHTED, HOST_WEIGHTED_EDGE = range(6)
+
+
+def host_match(csr: CSR, type_code: int) -> np.ndarray:
+    """One of the reference's sequential host matchers (types 0-5) run by the
+    library on the host; returns the match array in the reference encoding."""
+    r = np.ascontiguousarray(csr.ranges, np.int32)
+    nb = np.ascontiguousarray(csr.neighbours, np.int32)
+    w = None if csr.weights is None else np.ascontiguousarray(csr.weights, np.float32)
+    out = np.empty(csr.n, np.int32)
+    _check(lib().mm_host_match(type_code, csr.n, len(nb), _ptr(r), _ptr(nb), _ptr(w), _ptr(out)))
+    return out
 
 
 def test_matching(csr: CSR, match: np.ndarray):

@@ -18,6 +18,9 @@
  *   testMatching ........... src/matchcpu.cu:127-246
  *   CPU greedy types 0/4/5 . src/matchcpu.cu:259-303, 695-745, 777-810 (type 5
  *                            lives in mm_oracle_sort.cpp, it needs std::sort)
+ *   CPU types 2/3 .......... src/matchcpu.cu:518-682 (Karp-Sipser), 304-426
+ *                            (dynamic minimum degree); type 1 (static minimum
+ *                            degree, 428-516) is in mm_oracle_sort.cpp
  *
  * Every phase of a round writes only the slot of the vertex it processes and
  * reads only slots the phase does not write, so the OpenMP loops below give
@@ -37,6 +40,12 @@
  *   orc_get_weight / orc_test_matching: hand-computed known answers.
  *   orc_cpu_random / orc_cpu_weighted: hand-computed known answers and the
  *     textbook 1/2-approximation bound against brute-force maximum matchings.
+ *   orc_cpu_karp_sipser: maximum matchings on forests (Karp and Sipser's
+ *     degree-1 rule is optimal there; checked against a tree DP), hand-computed
+ *     known answers, validity and maximality.
+ *   orc_cpu_min_deg: closed forms (paths and cycles floor(n/2), stars 1,
+ *     complete bipartite min(a, b)), a hand-computed known answer that
+ *     exercises the recency order among equal degrees, validity, maximality.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -302,3 +311,118 @@ void orc_cpu_weighted(int32_t n, const int32_t *ranges, const int32_t *nbrs, con
         if (cand < n) match[i] = match[cand] = ORC_MATCH_VAL(i, cand);
     }
 }
+
+/* Type 2, one-sided Karp-Sipser: src/matchcpu.cu:518-682.  deg[] starts at
+ * the row lengths.  Singletons (degree exactly 1) are served first-in
+ * first-out: each is matched to its first unmatched neighbour, and every row
+ * entry of that partner lowers its target's degree.  With no singleton left,
+ * one pass over the remaining degree > 1 vertices, in vertex order (vertices
+ * never re-enter that set, so "in order" is a scan of ids with a membership
+ * flag), matches each still-unmatched vertex of nonzero degree to its first
+ * unmatched neighbour and lowers the targets of both rows; every vertex the
+ * pass reaches leaves the set.  Whenever an unmatched vertex's degree becomes
+ * exactly 1 it leaves the set and joins the singletons. */
+void orc_cpu_karp_sipser(int32_t n, const int32_t *ranges, const int32_t *nbrs, int32_t *match)
+{
+    int32_t *deg = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int32_t *queue = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * n + 1));
+    char *in_set = (char *)calloc((size_t)(n > 0 ? n : 1), 1);
+    int64_t q_head = 0, q_tail = 0, in_set_count = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        match[i] = 0;
+        deg[i] = ranges[2 * i + 1] - ranges[2 * i];
+        if (deg[i] == 1) queue[q_tail++] = i;
+        else if (deg[i] > 1) { in_set[i] = 1; ++in_set_count; }
+    }
+#define ORC_KS_LOWER(c)                                                              \
+    for (int32_t jj = ranges[2 * (c)]; jj < ranges[2 * (c) + 1]; ++jj) {             \
+        const int32_t t = nbrs[jj];                                                  \
+        deg[t] -= 1;                                                                 \
+        if (deg[t] == 1 && !ORC_IS_MATCHED(match[t])) {                              \
+            if (in_set[t]) { in_set[t] = 0; --in_set_count; }                        \
+            queue[q_tail++] = t;                                                     \
+        }                                                                            \
+    }
+    while (q_head < q_tail || in_set_count > 0) {
+        while (q_head < q_tail) {
+            const int32_t i = queue[q_head++];
+            if (ORC_IS_MATCHED(match[i]) || deg[i] < 1) continue;
+            int32_t cand = -1;
+            for (int32_t j = ranges[2 * i]; j < ranges[2 * i + 1] && cand < 0; ++j)
+                if (nbrs[j] != i && !ORC_IS_MATCHED(match[nbrs[j]])) cand = nbrs[j];
+            if (cand < 0) continue;
+            match[i] = match[cand] = ORC_MATCH_VAL(i, cand);
+            ORC_KS_LOWER(cand)
+        }
+        for (int32_t i = 0; i < n && in_set_count > 0; ++i) {
+            if (!in_set[i]) continue;
+            if (!ORC_IS_MATCHED(match[i]) && deg[i] != 0) {
+                int32_t cand = -1;
+                for (int32_t j = ranges[2 * i]; j < ranges[2 * i + 1] && cand < 0; ++j)
+                    if (nbrs[j] != i && !ORC_IS_MATCHED(match[nbrs[j]])) cand = nbrs[j];
+                if (cand >= 0) {
+                    match[i] = match[cand] = ORC_MATCH_VAL(i, cand);
+                    ORC_KS_LOWER(i)
+                    ORC_KS_LOWER(cand)
+                }
+            }
+            if (in_set[i]) { in_set[i] = 0; --in_set_count; }
+        }
+    }
+#undef ORC_KS_LOWER
+    free(deg);
+    free(queue);
+    free(in_set);
+}
+
+/* Type 3, two-sided dynamic minimum degree: src/matchcpu.cu:304-426.  The
+ * reference keeps (degree, vertex) in a std::multimap, where equal degrees
+ * come out in insertion order; here each queued vertex carries the time of
+ * its last (re)insertion and every step scans for the smallest (degree, time)
+ * -- quadratic, for small graphs.  Step: take that vertex i; its partner is
+ * the unmatched, queued neighbour of least degree (first in row order on
+ * ties); both leave the queue; then every row entry of i, and then of the
+ * partner, that names an unmatched queued vertex other than i lowers its
+ * degree by one (re-inserting it now, or dropping it at degree 0). */
+void orc_cpu_min_deg(int32_t n, const int32_t *ranges, const int32_t *nbrs, int32_t *match)
+{
+    int32_t *deg = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    int64_t *stamp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n > 0 ? n : 1));
+    char *queued = (char *)calloc((size_t)(n > 0 ? n : 1), 1);
+    int64_t clock = 0, left = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        match[i] = 0;
+        deg[i] = ranges[2 * i + 1] - ranges[2 * i];
+        if (deg[i] > 0) { queued[i] = 1; stamp[i] = clock++; ++left; }
+    }
+    while (left > 0) {
+        int32_t i = -1;
+        for (int32_t v = 0; v < n; ++v)
+            if (queued[v] && (i < 0 || deg[v] < deg[i] || (deg[v] == deg[i] && stamp[v] < stamp[i]))) i = v;
+        queued[i] = 0;
+        --left;
+        int32_t cand = -1;
+        for (int32_t j = ranges[2 * i]; j < ranges[2 * i + 1]; ++j) {
+            const int32_t u = nbrs[j];
+            if (u == i || ORC_IS_MATCHED(match[u]) || !queued[u]) continue;
+            if (cand < 0 || deg[u] < deg[cand]) cand = u;
+        }
+        if (cand < 0) continue;
+        match[i] = match[cand] = ORC_MATCH_VAL(i, cand);
+        queued[cand] = 0;
+        --left;
+        for (int side = 0; side < 2; ++side) {
+            const int32_t c = side == 0 ? i : cand;
+            for (int32_t j = ranges[2 * c]; j < ranges[2 * c + 1]; ++j) {
+                const int32_t u = nbrs[j];
+                if (u == i || ORC_IS_MATCHED(match[u]) || !queued[u]) continue;
+                if (deg[u] > 1) { deg[u] -= 1; stamp[u] = clock++; }
+                else { queued[u] = 0; --left; }
+            }
+        }
+    }
+    free(deg);
+    free(stamp);
+    free(queued);
+}
+

@@ -47,6 +47,9 @@ def lib():
         h.orc_cpu_random.argtypes = [ctypes.c_int32, _P, _P, _P]
         h.orc_cpu_weighted.restype = None
         h.orc_cpu_weighted.argtypes = [ctypes.c_int32, _P, _P, _P, _P]
+        for name in ("orc_cpu_stat_min_deg", "orc_cpu_karp_sipser", "orc_cpu_min_deg"):
+            getattr(h, name).restype = None
+            getattr(h, name).argtypes = [ctypes.c_int32, _P, _P, _P]
         h.orc_cpu_weighted_edge.restype = None
         h.orc_cpu_weighted_edge.argtypes = [ctypes.c_int32, ctypes.c_int64, _P, _P, _P, _P]
         h.orc_mlp_z_many.restype = None
@@ -136,6 +139,28 @@ def cpu_weighted(csr) -> np.ndarray:
     out = np.empty(csr.n, np.int32)
     lib().orc_cpu_weighted(csr.n, _p(ranges), _p(nb), _p(w), _p(out))
     return out
+
+
+def _unweighted_host(csr, fn_name) -> np.ndarray:
+    ranges, nb, _ = _csr_arrays(csr)
+    out = np.empty(csr.n, np.int32)
+    getattr(lib(), fn_name)(csr.n, _p(ranges), _p(nb), _p(out))
+    return out
+
+
+def cpu_stat_min_deg(csr) -> np.ndarray:
+    """Type 1, static minimum degree (src/matchcpu.cu:428-516)."""
+    return _unweighted_host(csr, "orc_cpu_stat_min_deg")
+
+
+def cpu_karp_sipser(csr) -> np.ndarray:
+    """Type 2, Karp-Sipser (src/matchcpu.cu:518-682)."""
+    return _unweighted_host(csr, "orc_cpu_karp_sipser")
+
+
+def cpu_min_deg(csr) -> np.ndarray:
+    """Type 3, dynamic minimum degree (src/matchcpu.cu:304-426); quadratic, small graphs only."""
+    return _unweighted_host(csr, "orc_cpu_min_deg")
 
 
 def edge_list(csr):

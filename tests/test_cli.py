@@ -1,5 +1,5 @@
 """The match command-line driver (bin/match.cpp's flags and output) on the
-CPU: the host matcher types 0, 4, 5, 10, 11 against the oracle, the output
+CPU: the host matcher types 0-5, 10, 11 against the oracle, the output
 layout, the gnuplot file and argument errors.  No GPU calls."""
 import os
 
@@ -23,7 +23,7 @@ def graph_file(tmp_path_factory):
 
 def test_host_types_match_the_oracle(graph_file):
     csr, path = graph_file
-    rc, out, err = run(["-m", "0 4 5 10 11", "-t", "-a", "2", "--threads", "1 3", path])
+    rc, out, err = run(["-m", "0 1 2 3 4 5 10 11", "-t", "-a", "2", "--threads", "1 3", path])
     assert rc == 0, err
     head, rows = parse(out)
     assert head == ["g3000.graph", "3000", str(csr.nnz // 2), str(0x88B81733)]
@@ -34,8 +34,9 @@ def test_host_types_match_the_oracle(graph_file):
         m2 = oracle.match(csr, weighted, True, reference_seeds(r1 + 255)[r1:])[0]
         return [m1, m2]
 
-    exp = {0: [oracle.cpu_random(csr)] * 2, 4: [oracle.cpu_weighted(csr)] * 2, 5: [oracle.cpu_weighted_edge(csr)] * 2,
-           10: two_reps(False), 11: two_reps(True)}
+    exp = {0: [oracle.cpu_random(csr)] * 2, 1: [oracle.cpu_stat_min_deg(csr)] * 2,
+           2: [oracle.cpu_karp_sipser(csr)] * 2, 3: [oracle.cpu_min_deg(csr)] * 2, 4: [oracle.cpu_weighted(csr)] * 2,
+           5: [oracle.cpu_weighted_edge(csr)] * 2, 10: two_reps(False), 11: two_reps(True)}
     for t, ms in exp.items():
         ws = [oracle.get_weight(csr, m) for m in ms]
         size = np.mean([x[1] for x in ws])
@@ -45,7 +46,7 @@ def test_host_types_match_the_oracle(graph_file):
             got = rows[(t, th)]
             assert got[0] == size, (t, th)
             assert abs(got[1] - w) <= 2e-5 * w, (t, th)   # printed with 6 significant digits
-    assert out.count("0 can still be matched.") == 14   # -t: every result verified and maximal
+    assert out.count("0 can still be matched.") == 20   # -t: every result verified and maximal
 
 
 def test_gnuplot_layout(graph_file, tmp_path):
@@ -68,10 +69,10 @@ def test_argument_errors(args, msg):
     assert rc == 255 and msg in err
 
 
-def test_types_outside_this_build_are_refused(graph_file):
+def test_unknown_types_are_refused(graph_file):
     _, path = graph_file
-    rc, out, err = run(["-m", "2", path])
-    assert rc == 255 and "not part of this build" in err
+    rc, out, err = run(["-m", "12", path])
+    assert rc == 255 and "Unknown matching type!" in err
 
 
 def test_random_shuffle_changes_order_not_validity(graph_file):
