@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 
 #include <iostream>
+#include <memory>
+#include <string>
 #include <vector>
 
 namespace mtc
@@ -44,6 +46,12 @@ class Graph
         std::vector<float> neighbourWeights;
         std::vector<Edge> edges;
 };
+
+// Opens a graph file for readMETIS / readMatrixMarket.  A name containing
+// ".bz2" is decompressed while it is read (bin/match.cpp:85-110 reads such
+// files through a bzip2 filter); libbz2 is loaded at run time.  Returns null
+// and sets error when the file cannot be opened.
+std::unique_ptr<std::istream> openGraphInput(const std::string &fileName, std::string &error);
 
 }  // namespace mtc
 

@@ -8,7 +8,16 @@
 //     matrices only; entries strictly below the diagonal become edges with
 //     weight |z| = sqrt(re^2 + im^2); each edge is appended to both rows in
 //     edge order.
+// Compressed inputs (bin/match.cpp:81-110 reads *.graph.bz2 and *.mtx.bz2
+// through a bzip2 filter) are decompressed while they are read, by libbz2's
+// public streaming API, loaded when the first such file is opened.
+#include <dlfcn.h>
+
 #include <cctype>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <mutex>
 #include <cmath>
 #include <exception>
 #include <iostream>
@@ -40,7 +49,140 @@ string strip_leading(string line)
     return line.substr(k);
 }
 
+// bzip2 streaming decompression (libbz2's documented low-level interface:
+// a bz_stream, BZ2_bzDecompressInit / BZ2_bzDecompress / BZ2_bzDecompressEnd).
+struct BzStream {
+    char *next_in;
+    unsigned int avail_in, total_in_lo32, total_in_hi32;
+    char *next_out;
+    unsigned int avail_out, total_out_lo32, total_out_hi32;
+    void *state;
+    void *(*bzalloc)(void *, int, int);
+    void (*bzfree)(void *, void *);
+    void *opaque;
+};
+constexpr int BZ_OK_ = 0, BZ_STREAM_END_ = 4;
+
+struct Bz2Api {
+    int (*init)(BzStream *, int, int) = nullptr;
+    int (*run)(BzStream *) = nullptr;
+    int (*end)(BzStream *) = nullptr;
+};
+
+const Bz2Api *bz2_api()
+{
+    static Bz2Api api;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        void *h = dlopen("libbz2.so.1", RTLD_NOW | RTLD_LOCAL);
+        if (!h) h = dlopen("libbz2.so.1.0", RTLD_NOW | RTLD_LOCAL);
+        if (!h) return;
+        api.init = reinterpret_cast<int (*)(BzStream *, int, int)>(dlsym(h, "BZ2_bzDecompressInit"));
+        api.run = reinterpret_cast<int (*)(BzStream *)>(dlsym(h, "BZ2_bzDecompress"));
+        api.end = reinterpret_cast<int (*)(BzStream *)>(dlsym(h, "BZ2_bzDecompressEnd"));
+    });
+    return api.init && api.run && api.end ? &api : nullptr;
+}
+
+// A read-only streambuf over a .bz2 file: compressed bytes in 1 MiB chunks,
+// decompressed into a 4 MiB window on each underflow.  Concatenated bzip2
+// streams (as written by parallel compressors) are read one after another.
+class Bz2Buf : public std::streambuf
+{
+    public:
+        Bz2Buf(const string &path, const Bz2Api *api) : api_(api), file_(path.c_str(), ios_base::binary), in_(1 << 20), out_(4 << 20)
+        {
+            memset(&bz_, 0, sizeof(bz_));
+            ok_ = file_.good() && api_->init(&bz_, 0, 0) == BZ_OK_;
+            open_ = ok_;
+        }
+        ~Bz2Buf() override
+        {
+            if (open_) api_->end(&bz_);
+        }
+        bool ok() const { return ok_; }
+
+    protected:
+        int_type underflow() override
+        {
+            if (gptr() < egptr()) return traits_type::to_int_type(*gptr());
+            while (ok_) {
+                if (bz_.avail_in == 0 && !eof_) {
+                    file_.read(in_.data(), (streamsize)in_.size());
+                    const streamsize got = file_.gcount();
+                    if (got <= 0) eof_ = true;
+                    bz_.next_in = in_.data();
+                    bz_.avail_in = (unsigned int)max<streamsize>(got, 0);
+                }
+                if (!open_) {
+                    // Between streams: more input starts another one.
+                    if (bz_.avail_in == 0) return traits_type::eof();
+                    if (api_->init(&bz_, 0, 0) != BZ_OK_) { ok_ = false; break; }
+                    open_ = true;
+                }
+                bz_.next_out = out_.data();
+                bz_.avail_out = (unsigned int)out_.size();
+                const int rc = api_->run(&bz_);
+                const size_t produced = out_.size() - bz_.avail_out;
+                if (rc == BZ_STREAM_END_) {
+                    api_->end(&bz_);
+                    open_ = false;
+                } else if (rc != BZ_OK_ || (produced == 0 && bz_.avail_in == 0 && eof_)) {
+                    // Corrupt or truncated input: the istream turns this into badbit.
+                    ok_ = false;
+                    throw ios_base::failure("corrupt bzip2 input");
+                }
+                if (produced > 0) {
+                    setg(out_.data(), out_.data(), out_.data() + produced);
+                    return traits_type::to_int_type(*gptr());
+                }
+                if (!open_ && bz_.avail_in == 0 && eof_) return traits_type::eof();
+            }
+            return traits_type::eof();
+        }
+
+    private:
+        const Bz2Api *api_;
+        ifstream file_;
+        vector<char> in_, out_;
+        BzStream bz_;
+        bool ok_ = false, open_ = false, eof_ = false;
+};
+
+class Bz2Istream : public std::istream
+{
+    public:
+        Bz2Istream(const string &path, const Bz2Api *api) : std::istream(nullptr), buf_(path, api) { rdbuf(&buf_); }
+        bool ok() const { return buf_.ok(); }
+
+    private:
+        Bz2Buf buf_;
+};
+
 }  // namespace
+
+std::unique_ptr<std::istream> openGraphInput(const std::string &fileName, std::string &error)
+{
+    if (fileName.find(".bz2") == string::npos) {
+        std::unique_ptr<std::istream> in(new ifstream(fileName.c_str()));
+        if (!in->good()) {
+            error = "Unable to open '" + fileName + "'!";
+            return nullptr;
+        }
+        return in;
+    }
+    const Bz2Api *api = bz2_api();
+    if (!api) {
+        error = "Unable to load libbz2 to read '" + fileName + "'!";
+        return nullptr;
+    }
+    std::unique_ptr<Bz2Istream> in(new Bz2Istream(fileName, api));
+    if (!in->ok()) {
+        error = "Unable to open '" + fileName + "' as a bzip2 file!";
+        return nullptr;
+    }
+    return std::unique_ptr<std::istream>(in.release());
+}
 
 istream &Graph::readMETIS(istream &in)
 {

@@ -80,3 +80,45 @@ def test_random_shuffle_changes_order_not_validity(graph_file):
     rc, out, err = run(["-m", "4 11", "-r", "-t", "-a", "3", path])
     assert rc == 0, err
     assert "Invalid matching" not in err
+
+
+@pytest.mark.parametrize("ext", [".graph", ".mtx"])
+def test_bzip2_inputs_read_like_plain_files(graph_file, tmp_path, ext):
+    """*.graph.bz2 / *.mtx.bz2 (bin/match.cpp:85-110): decompressed while read,
+    also when the file holds several concatenated bzip2 streams; the reported
+    name drops the .bz2 suffix."""
+    import bz2
+
+    csr, path = graph_file
+    if ext == ".mtx":
+        r = csr.ranges
+        lines = ["%%MatrixMarket matrix coordinate real symmetric", f"{csr.n} {csr.n} {csr.nnz // 2}"]
+        for i in range(csr.n):
+            for j in range(r[i, 0], r[i, 1]):
+                if csr.neighbours[j] < i:
+                    lines.append(f"{i + 1} {csr.neighbours[j] + 1} {float(csr.weights[j]):.9g}")
+        text = ("\n".join(lines) + "\n").encode()
+    else:
+        text = open(path, "rb").read()
+    plain = str(tmp_path / ("g" + ext))
+    open(plain, "wb").write(text)
+    one = str(tmp_path / ("one" + ext + ".bz2"))
+    open(one, "wb").write(bz2.compress(text))
+    multi = str(tmp_path / ("multi" + ext + ".bz2"))
+    cut = len(text) // 3
+    open(multi, "wb").write(bz2.compress(text[:cut]) + bz2.compress(text[cut:2 * cut]) + bz2.compress(text[2 * cut:]))
+    outs = []
+    for f in (plain, one, multi):
+        rc, out, err = run(["-m", "0 4 5", "-t", f])
+        assert rc == 0, err
+        head, rows = parse(out)
+        outs.append((head[1:], {k: v[:2] for k, v in rows.items()}))  # size, weight (not the timings)
+        assert head[0] == os.path.basename(f).replace(".bz2", "")
+    assert outs[0] == outs[1] == outs[2]
+
+
+def test_corrupt_bzip2_input_is_an_error(tmp_path):
+    bad = str(tmp_path / "bad.graph.bz2")
+    open(bad, "wb").write(b"BZh91AY&SY" + bytes(range(200)))
+    rc, out, err = run(["-m", "4", bad])
+    assert rc == 255
