@@ -104,31 +104,37 @@ def load_params(path: str = DEFAULT_PARAMS) -> np.ndarray:
 
 def compare(csr, params, seeds=None, graph=None, max_rounds: int = 0) -> dict:
     """Plain and learned maximal matchings of one graph on the device:
-    rounds, matched vertices, weight, device time, and the first round at
+    rounds, matched vertices, weight, device time (result left in HBM), and the first round at
     which the learned run's weight reaches the plain run's final weight."""
     if seeds is None:
         seeds = mmamd.reference_seeds(mmamd.MM_MAX_ROUNDS)
     g = graph if graph is not None else mmamd.Graph(csr)
     # untimed first runs: buffers and first kernel launches
-    g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
-    g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
-    plain = g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
-    learned = g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
-    wp, sp = mmamd.get_weight(csr, plain.match)
-    wl, sl = mmamd.get_weight(csr, learned.match)
+    g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds, fetch=False)
+    g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds, fetch=False)
+    # timed runs leave the result on the device (match_ms is the device time of
+    # the rounds and the unpermute); the arrays are copied out afterwards
+    plain = g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds, fetch=False)
+    plain_match = g.copy_match()
+    learned = g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds,
+                              fetch=False)
+    learned_match = g.copy_match()
+    wp, sp = mmamd.get_weight(csr, plain_match)
+    wl, sl = mmamd.get_weight(csr, learned_match)
     reach, reach_ms = None, None
     for r in range(1, learned.rounds + 1):
-        m = g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=False, seeds=seeds, max_rounds=r)
-        if mmamd.get_weight(csr, m.match)[0] >= wp:
+        m = g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=False, seeds=seeds, max_rounds=r, fetch=False)
+        if mmamd.get_weight(csr, g.copy_match())[0] >= wp:
             reach, reach_ms = r, m.match_ms
             break
     # the same kernels with the plain colouring: the cost of the learned rounds without the network
-    through = g.match_learned(None, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds)
+    through = g.match_learned(None, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds,
+                              fetch=False)
     out = {
         "plain": {"rounds": plain.rounds, "matched": sp, "weight": wp, "ms": plain.match_ms,
-                  "valid": mmamd.test_matching(csr, plain.match)[0]},
+                  "valid": mmamd.test_matching(csr, plain_match)[0]},
         "learned": {"rounds": learned.rounds, "matched": sl, "weight": wl, "ms": learned.match_ms,
-                    "valid": mmamd.test_matching(csr, learned.match)[0],
+                    "valid": mmamd.test_matching(csr, learned_match)[0],
                     "rounds_to_plain_weight": reach, "ms_to_plain_weight": reach_ms},
         "plain_colouring_in_learned_kernels_ms": through.match_ms,
         "weight_ratio": wl / wp if wp else None,
