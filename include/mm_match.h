@@ -121,6 +121,12 @@ int32_t mm_graph_create(int32_t device, int32_t n, int64_t nnz, const int32_t *r
  * memory; results, hash inputs and tie-breaks still use the caller's ids.
  * MM_CREATE_KEEP_ORDER keeps the caller's numbering on the device. */
 #define MM_CREATE_KEEP_ORDER 1u
+/* MM_CREATE_PACKED_ROWS stores the device rows back to back.  By default a row
+ * whose first 8 entries would straddle a 128-byte line starts at the next line
+ * (about 1.75 entries of padding per row on average), so scans of scattered
+ * rows in the later rounds fetch one line per row step.  Results are the same
+ * either way. */
+#define MM_CREATE_PACKED_ROWS 2u
 int32_t mm_graph_create_ex(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges,
                            const int32_t *neighbours, const float *weights, uint32_t flags, mm_graph **out);
 int32_t mm_graph_destroy(mm_graph *g);

@@ -1071,7 +1071,7 @@ std::vector<int32_t> bfs_order(int32_t n, const int32_t *ranges, const int32_t *
 // vbase_old: per ORIGINAL vertex, the first id of its batched graph (null for
 // a single graph).  relabel: renumber the vertices internally (bfs_order).
 int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *nbrs,
-                     const float *w, const int32_t *vbase_old, int relabel, mm_graph **out)
+                     const float *w, const int32_t *vbase_old, int relabel, bool align_rows, mm_graph **out)
 {
     mm::clear_error();
     if (!out) return mm::set_error(MM_ERR_INVALID, "mm_graph_create: out is NULL");
@@ -1107,8 +1107,6 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         return fail(mm::set_error(MM_ERR_HIP, "hipEventCreate failed"));
     const size_t nn = (size_t)n;
     if ((rc = dev_alloc(g, &g->d_ranges, nn, "neighbour ranges")) != MM_OK) return fail(rc);
-    if ((rc = dev_alloc(g, &g->d_nbrs, (size_t)nnz, "neighbours")) != MM_OK) return fail(rc);
-    if (g->weighted && (rc = dev_alloc(g, &g->d_adj, (size_t)nnz, "sorted weighted rows")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_match, nn, "match")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_req, nn, "requests")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_st, nn, "state")) != MM_OK) return fail(rc);
@@ -1151,21 +1149,37 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     }
     auto old_of = [&](int64_t i) -> int32_t { return g->relabelled ? order[(size_t)i] : (int32_t)i; };
     std::vector<int32_t> h_ranges(2 * nn), h_loc(nn);
-    int64_t off = 0;
     std::vector<int32_t> h_vbase;
     if (vbase_old) h_vbase.resize(nn);
+    // Row offsets of the device copies.  With align_rows a row whose first 8
+    // entries would straddle a 128-byte line of adj (16 entries) starts at the
+    // next line instead, so the first step of a scan over a lone row costs one
+    // line; the gap is never read.  Offsets must stay int32.
+    int64_t nnz_dev = 0;
+    for (int pass = align_rows ? 0 : 1; pass < 2; ++pass) {
+        int64_t off = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const int32_t o = old_of(i);
+            const int32_t deg = ranges[2 * (size_t)o + 1] - ranges[2 * (size_t)o];
+            if (pass == 0 && (off & 15) + std::min<int64_t>(deg, 8) > 16) off = (off + 15) & ~(int64_t)15;
+            h_ranges[2 * (size_t)i] = (int32_t)off;
+            h_ranges[2 * (size_t)i + 1] = (int32_t)(off + deg);
+            off += deg;
+        }
+        nnz_dev = off;
+        if (off <= INT32_MAX) break;
+    }
     for (int32_t i = 0; i < n; ++i) {
         const int32_t o = old_of(i);
         if (vbase_old) h_vbase[(size_t)i] = vbase_old[(size_t)o];
-        const int32_t deg = ranges[2 * (size_t)o + 1] - ranges[2 * (size_t)o];
-        h_ranges[2 * (size_t)i] = (int32_t)off;
-        h_ranges[2 * (size_t)i + 1] = (int32_t)(off + deg);
         h_loc[(size_t)i] = o - (vbase_old ? vbase_old[(size_t)o] : 0);
-        off += deg;
     }
-    std::vector<int32_t> h_nbrs((size_t)nnz);
+    if ((rc = dev_alloc(g, &g->d_nbrs, (size_t)nnz_dev, "neighbours")) != MM_OK) return fail(rc);
+    if (g->weighted && (rc = dev_alloc(g, &g->d_adj, (size_t)nnz_dev, "sorted weighted rows")) != MM_OK)
+        return fail(rc);
+    std::vector<int32_t> h_nbrs((size_t)nnz_dev, 0);
     std::vector<int2> h_adj;
-    if (g->weighted) h_adj.resize((size_t)nnz);
+    if (g->weighted) h_adj.assign((size_t)nnz_dev, make_int2(0, 0));
     mm::parallel_for_dynamic(n, 1024, [&](int64_t b, int64_t e, unsigned) {
         std::vector<int32_t> perm;
         for (int64_t i = b; i < e; ++i) {
@@ -1198,10 +1212,11 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     if (n > 0 && (hipMemcpy(g->d_ranges, h_ranges.data(), sizeof(int2) * nn, hipMemcpyHostToDevice) != hipSuccess ||
                   hipMemcpy(g->d_loc, h_loc.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess))
         return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
-    if (nnz > 0) {
-        if (hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess ||
+    if (nnz_dev > 0) {
+        if (hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz_dev, hipMemcpyHostToDevice) !=
+                hipSuccess ||
             (g->weighted &&
-             hipMemcpy(g->d_adj, h_adj.data(), sizeof(int2) * (size_t)nnz, hipMemcpyHostToDevice) != hipSuccess))
+             hipMemcpy(g->d_adj, h_adj.data(), sizeof(int2) * (size_t)nnz_dev, hipMemcpyHostToDevice) != hipSuccess))
             return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
     }
     if (vbase_old && n > 0) {
@@ -1762,15 +1777,16 @@ int32_t mm_device_count(void)
 int32_t mm_graph_create(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *neighbours,
                         const float *weights, mm_graph **out)
 {
-    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, 1, out);
+    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, 1, true, out);
 }
 
 int32_t mm_graph_create_ex(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *neighbours,
                            const float *weights, uint32_t flags, mm_graph **out)
 {
-    if (flags & ~(uint32_t)MM_CREATE_KEEP_ORDER) return mm::set_error(MM_ERR_INVALID, "mm_graph_create_ex: bad flags");
-    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr,
-                        (flags & MM_CREATE_KEEP_ORDER) ? 0 : 1, out);
+    if (flags & ~(uint32_t)(MM_CREATE_KEEP_ORDER | MM_CREATE_PACKED_ROWS))
+        return mm::set_error(MM_ERR_INVALID, "mm_graph_create_ex: bad flags");
+    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, (flags & MM_CREATE_KEEP_ORDER) ? 0 : 1,
+                        (flags & MM_CREATE_PACKED_ROWS) == 0, out);
 }
 
 int32_t mm_graph_destroy(mm_graph *g)
@@ -1853,7 +1869,7 @@ int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_
     if (bad) return mm::set_error(MM_ERR_INVALID, "mm_batch_create: malformed per-graph CSR");
     mm_graph *g = nullptr;
     int32_t rc = create_graph(device, (int32_t)total_n, total_nnz, granges.data(), gnbrs.data(), weights,
-                              vbase.data(), 1, &g);
+                              vbase.data(), 1, true, &g);
     if (rc != MM_OK) return rc;
     g->n_graphs = n_graphs;
     if ((rc = dev_alloc(g, &g->d_voff, (size_t)n_graphs + 1, "batch offsets")) != MM_OK ||
