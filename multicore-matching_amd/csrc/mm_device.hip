@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <climits>
 #include <cstdint>
 #include <cstring>
@@ -1487,7 +1488,10 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     const int R_run = maximal ? std::min(R_max, o->n_seeds) : R_max;
     // Fixed runs are one launch sequence; maximal runs check the round
     // counters after every chunk of rounds (rounds past convergence are no-ops).
-    const int chunk = maximal ? 4 : R_run;
+    // The first chunk is 4 rounds; later ones last as long as the decay of the
+    // active count predicts the run still needs, so a typical run reads the
+    // counters back twice.
+    int chunk = maximal ? 4 : R_run;
     auto enq_reduce = [&](int r0, int r1) -> int32_t {
         MM_LAUNCH(K_CLEAR, hipLaunchKernelGGL(k_reduce_counts, dim3(r1 - r0 + 1), dim3(256), 0, g->stream, d, r0,
                                               d.count_nnz));
@@ -1565,11 +1569,14 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
                                           hipMemcpyDeviceToHost, g->stream),
                            "Unable to read round counters");
                 MM_HIP_TRY(hipStreamSynchronize(g->stream), "A HIP error occurred during the matching process");
-                for (int k = r; k <= k_hi; ++k) {
+                auto active = [&](int k) {
                     const int32_t *q = g->h_ctr + (size_t)k * CTR_STRIDE;
-                    if ((int64_t)q[0] + q[1] + q[2] + q[3] == 0) { rounds = k; break; }
-                }
+                    return (int64_t)q[0] + q[1] + q[2] + q[3];
+                };
+                for (int k = r; k <= k_hi; ++k)
+                    if (active(k) == 0) { rounds = k; break; }
                 if (rounds > 0) break;
+                chunk = predicted_rounds(k_hi > r ? active(k_hi - 1) : 0, active(k_hi));
             }
             r = end + 1;
         }
@@ -1751,6 +1758,20 @@ int32_t mm_graph_alloc(mm_graph *g, void **p, size_t bytes, const char *what)
     const int32_t rc = dev_alloc(g, &q, bytes, what);
     *p = q;
     return rc;
+}
+
+// Rounds a maximal run is expected to need after a round with `now` active
+// vertices, when the round before had `before`: the active count shrinks by a
+// roughly constant factor per round, so log(now) / log(before / now) rounds
+// reach zero.  Clamped to [2, 32]; a short guess costs one more read-back, a
+// long one a few empty rounds (a few microseconds each).
+int predicted_rounds(int64_t before, int64_t now)
+{
+    if (now <= 0) return 2;
+    if (before <= now) return 8;
+    const double shrink = std::log((double)before / (double)now);
+    const double need = std::log((double)now + 1.0) / std::max(shrink, 1e-3) + 2.0;
+    return (int)std::min(32.0, std::max(2.0, std::ceil(need)));
 }
 
 void mm_graph_enqueue_unpermute(mm_graph *g)

@@ -93,5 +93,8 @@ int32_t mm_graph_alloc(mm_graph *g, void **p, size_t bytes, const char *what);
 // Enqueues the copy of the internal match array into the caller's vertex order
 // (relabelled graphs only; a no-op otherwise).
 void mm_graph_enqueue_unpermute(mm_graph *g);
+// Rounds a maximal run should run next, from the active counts of its last
+// two rounds (before, now); the chunk length between counter read-backs.
+int predicted_rounds(int64_t before, int64_t now);
 
 #endif

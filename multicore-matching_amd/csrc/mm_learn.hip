@@ -986,6 +986,9 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
     hipLaunchKernelGGL(k_l_fill, dim3(1), dim3(64), 0, st, (int32_t *)d.scount, (int64_t)4, 0);
     if (g->n > 0) hipLaunchKernelGGL(k_l_start, dim3(grid), dim3(block), 0, st, d.match, d.st, (int64_t)g->n);
     const int R_run = maximal ? std::min(R_max, o->n_seeds) : R_max;
+    // Maximal runs read the counters back every 4 rounds: an empty learned round
+    // still sweeps the select (about as costly as a read-back), so the plain
+    // rounds' longer predicted chunks do not pay off here.
     const int chunk = maximal ? 4 : R_run;
     int rounds = -1;
     for (int r = 1; r <= R_run && g->n > 0;) {
