@@ -48,6 +48,7 @@ constexpr uint8_t LC_BLUE = 0, LC_RED = 1, LC_RETIRE = 2, LC_DONE = 3;
 // (match[v] in one byte, the form neighbour gathers read).
 constexpr int SEL_G = 8;   // lanes per row in the select's feature scans
 constexpr int SCAN_G = 4;  // lanes per row in the request / respond scans
+constexpr int SCAN_ROWS = 4;  // rows each select group scans at once
 
 static_assert(MM_MLP_OFF_B3 + 1 == NP, "parameter layout");
 
@@ -166,6 +167,8 @@ struct LDev {
     unsigned long long *resp;  // PUSH proposal keys, one per vertex
     int32_t *lact, *lblue;     // work lists of the round: block b owns [b * seg, (b + 1) * seg)
     int32_t *lcnt;             // [2][LPART] per-block list lengths (active, blue)
+    const int32_t *lprev;      // the previous round's active list (same segments) ...
+    const int32_t *lcnt_prev;  // ... and its per-block lengths
     int64_t seg;
 };
 
@@ -344,14 +347,24 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
     int32_t *act_seg = d.lact + (size_t)blockIdx.x * d.seg;
     int32_t *blue_seg = d.lblue + (size_t)blockIdx.x * d.seg;
     int n_act = 0, n_blue = 0;
-    for (int64_t base = (int64_t)blockIdx.x * STILE; base < d.n; base += (int64_t)gridDim.x * STILE) {
+    // Round 1 takes tiles b, b + gridDim, ... of the ids; later rounds take the
+    // vertices block b listed as active in the round before (a superset of the
+    // active ones, in ascending order), so a round costs what its survivors cost.
+    const bool from_list = r > 1;
+    const int32_t *prev = d.lprev + (size_t)blockIdx.x * d.seg;
+    const int64_t limit = from_list ? (int64_t)d.lcnt_prev[blockIdx.x] : (int64_t)d.n;
+    const int64_t stride = from_list ? (int64_t)STILE : (int64_t)gridDim.x * STILE;
+    for (int64_t base = from_list ? 0 : (int64_t)blockIdx.x * STILE; base < limit; base += stride) {
+        int32_t c0 = -1, c1 = -1;
+        if (base + t < limit) c0 = from_list ? prev[base + t] : (int32_t)(base + t);
+        if (base + LTILE + t < limit) c1 = from_list ? prev[base + LTILE + t] : (int32_t)(base + LTILE + t);
         int A0, A1;
-        const bool act0 = base + t < d.n && d.match[base + t] < 2;
-        const bool act1 = base + LTILE + t < d.n && d.match[base + LTILE + t] < 2;
+        const bool act0 = c0 >= 0 && d.match[c0] < 2;
+        const bool act1 = c1 >= 0 && d.match[c1] < 2;
         const int pos0 = block_prefix(act0, A0);
-        if (act0) s_idx[pos0] = t;
+        if (act0) s_idx[pos0] = c0;
         const int pos1 = block_prefix(act1, A1);
-        if (act1) s_idx[A0 + pos1] = LTILE + t;
+        if (act1) s_idx[A0 + pos1] = c1;
         const int A = A0 + A1;
         __syncthreads();
         if (scan) {
@@ -361,7 +374,7 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
                 // two candidates are its first two entries that can win (rows
                 // keep those first).
                 for (int k = t; k < A; k += LTILE) {
-                    const int2 rg = d.ranges[base + s_idx[k]];
+                    const int2 rg = d.ranges[s_idx[k]];
                     int32_t p1 = INT_MAX, p2 = INT_MAX;
                     for (int32_t j = rg.x; j < rg.y && j < rg.x + 2; ++j) {
                         int32_t u;
@@ -378,32 +391,58 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
                     s_redp[k] = 0;
                 }
             } else
-            for (int k = grp; k < A; k += GROUPS) {
-                const int64_t v = base + s_idx[k];
-                const int2 rg = d.ranges[v];
-                int alive = 0, redp = 0, p1 = INT_MAX, p2 = INT_MAX;
-                for (int32_t j = rg.x + gl; j < rg.y; j += G) {
-                    int32_t u;
-                    bool valid;
-                    float w;
-                    l_entry<A8>(d, j, u, valid, w);
-                    const int su = d.st[u];
-                    if (su >= 3) continue;
-                    ++alive;
-                    redp += su == 1;
-                    if (valid) {
-                        if (p1 == INT_MAX) p1 = j;
-                        else if (p2 == INT_MAX) p2 = j;
+            // SCAN_ROWS rows per group at once: their entry loads, then their
+            // state gathers, are in flight together.
+            for (int k0 = grp; k0 < A; k0 += GROUPS * SCAN_ROWS) {
+                int2 rg[SCAN_ROWS];
+                int alive[SCAN_ROWS], redp[SCAN_ROWS], p1[SCAN_ROWS], p2[SCAN_ROWS];
+                int32_t maxlen = 0;
+#pragma unroll
+                for (int q = 0; q < SCAN_ROWS; ++q) {
+                    const int k = k0 + q * GROUPS;
+                    rg[q] = k < A ? d.ranges[s_idx[k]] : make_int2(0, 0);
+                    maxlen = max(maxlen, rg[q].y - rg[q].x);
+                    alive[q] = redp[q] = 0;
+                    p1[q] = p2[q] = INT_MAX;
+                }
+                for (int32_t off = gl; off < maxlen; off += G) {
+                    int32_t u[SCAN_ROWS];
+                    bool valid[SCAN_ROWS], in[SCAN_ROWS];
+#pragma unroll
+                    for (int q = 0; q < SCAN_ROWS; ++q) {
+                        in[q] = off < rg[q].y - rg[q].x;
+                        float w;
+                        u[q] = 0;
+                        valid[q] = false;
+                        if (in[q]) l_entry<A8>(d, rg[q].x + off, u[q], valid[q], w);
+                    }
+                    int su[SCAN_ROWS];
+#pragma unroll
+                    for (int q = 0; q < SCAN_ROWS; ++q) su[q] = in[q] ? (int)d.st[u[q]] : 3;
+#pragma unroll
+                    for (int q = 0; q < SCAN_ROWS; ++q) {
+                        if (su[q] >= 3) continue;
+                        ++alive[q];
+                        redp[q] += su[q] == 1;
+                        if (valid[q]) {
+                            const int32_t j = rg[q].x + off;
+                            if (p1[q] == INT_MAX) p1[q] = j;
+                            else if (p2[q] == INT_MAX) p2[q] = j;
+                        }
                     }
                 }
-                alive = group_sum<G>(alive);
-                redp = group_sum<G>(redp);
-                group_min2<G>(p1, p2);
-                if (gl == 0) {
-                    s_p1[k] = p1;
-                    s_p2[k] = p2;
-                    s_alive[k] = alive;
-                    s_redp[k] = redp;
+#pragma unroll
+                for (int q = 0; q < SCAN_ROWS; ++q) {
+                    const int k = k0 + q * GROUPS;
+                    const int a = group_sum<G>(alive[q]);
+                    const int rp = group_sum<G>(redp[q]);
+                    group_min2<G>(p1[q], p2[q]);
+                    if (gl == 0 && k < A) {
+                        s_p1[k] = p1[q];
+                        s_p2[k] = p2[q];
+                        s_alive[k] = a;
+                        s_redp[k] = rp;
+                    }
                 }
             }
             __syncthreads();
@@ -416,11 +455,11 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
         o0.c = o1.c = LC_RED;
         o0.need_net = o1.need_net = false;
         if (has0) {
-            v0 = (int32_t)(base + s_idx[t]);
+            v0 = s_idx[t];
             sel_features<A8>(d, r, seed, scan, v0, s_p1, s_p2, s_alive, s_redp, t, vl0, o0);
         }
         if (has1) {
-            v1 = (int32_t)(base + s_idx[t + LTILE]);
+            v1 = s_idx[t + LTILE];
             sel_features<A8>(d, r, seed, scan, v1, s_p1, s_p2, s_alive, s_redp, t + LTILE, vl1, o1);
         }
         if (o0.need_net && o1.need_net) {
@@ -652,15 +691,21 @@ __global__ __launch_bounds__(LTILE) void k_l_match(LDev d, int r, uint32_t seed)
 }
 
 // ctr[r] = {active, blue, newly matched, newly dead} from the round's per-block counts.
-__global__ void k_l_reduce(const int32_t *lcnt, const int32_t *part, int nblk, int32_t *ctr_r)
+__global__ __launch_bounds__(256 * LCTR) void k_l_reduce(const int32_t *lcnt, const int32_t *part, int nblk,
+                                                         int32_t *ctr_r)
 {
-    const int c = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // 256 threads per counter, four loads in flight per thread.
+    __shared__ int ws[4 * LCTR];
+    const int c = threadIdx.x >> 8, t = threadIdx.x & 255;
     const int32_t *src = c < 2 ? lcnt + c * LPART : part + c * LPART;
     int x = 0;
-    for (int b = lane; b < nblk; b += 64) x += src[b];
+#pragma unroll 4
+    for (int b = t; b < nblk; b += 256) x += src[b];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    if (lane == 0) ctr_r[c] = x;
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (t == 0) ctr_r[c] = ws[4 * c] + ws[4 * c + 1] + ws[4 * c + 2] + ws[4 * c + 3];
 }
 
 // ---------------------------------------------------------------------------
@@ -922,7 +967,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         return rc;
     if (trace && (rc = graph_buf(g, &g->d_lfeat, nn * NF, "features")) != MM_OK) return rc;
     if ((rc = graph_buf(g, &g->d_lpart, (size_t)LCTR * LPART, "partial counts")) != MM_OK) return rc;
-    if ((rc = graph_buf(g, &g->d_lcnt, (size_t)2 * LPART, "list lengths")) != MM_OK) return rc;
+    if ((rc = graph_buf(g, &g->d_lcnt, (size_t)4 * LPART, "list lengths")) != MM_OK) return rc;
     if (!g->h_lctr && hipHostMalloc((void **)&g->h_lctr, sizeof(int32_t) * LSLOTS * LCTR) != hipSuccess)
         return mm::set_error(MM_ERR_NOMEM, "pinned host allocation failed");
 
@@ -967,20 +1012,22 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
     const int64_t tiles = ((int64_t)g->n + STILE - 1) / STILE;
     const int sgrid = (int)std::max<int64_t>(1, std::min<int64_t>({tiles, (int64_t)g->num_cu * 8, LPART}));
     d.seg = (tiles + sgrid - 1) / sgrid * STILE;
-    if ((size_t)(2 * sgrid * d.seg) > g->llist_cap) {
+    // Lists: active (by round parity, the select reads the previous round's)
+    // and blue; list lengths likewise by round parity.
+    if ((size_t)(3 * sgrid * d.seg) > g->llist_cap) {
         if (g->d_llist) (void)hipFree(g->d_llist);
         g->d_llist = nullptr;
         g->llist_cap = 0;
         void *q = nullptr;
-        if ((rc = mm_graph_alloc(g, &q, sizeof(int32_t) * 2 * (size_t)sgrid * (size_t)d.seg, "learned work lists")) !=
+        if ((rc = mm_graph_alloc(g, &q, sizeof(int32_t) * 3 * (size_t)sgrid * (size_t)d.seg, "learned work lists")) !=
             MM_OK)
             return rc;
         g->d_llist = (int32_t *)q;
-        g->llist_cap = (size_t)(2 * sgrid * d.seg);
+        g->llist_cap = (size_t)(3 * sgrid * d.seg);
     }
-    d.lact = g->d_llist;
-    d.lblue = g->d_llist + (size_t)sgrid * d.seg;
-    d.lcnt = g->d_lcnt;
+    int32_t *act_list[2] = {g->d_llist, g->d_llist + (size_t)sgrid * d.seg};
+    int32_t *cnt_list[2] = {g->d_lcnt, g->d_lcnt + 2 * LPART};
+    d.lblue = g->d_llist + 2 * (size_t)sgrid * d.seg;
     ML_TRY(hipEventRecord(g->ev_t1, st), "hipEventRecord");
     hipLaunchKernelGGL(k_l_fill, dim3(1), dim3(256), 0, st, d.ctr, (int64_t)LSLOTS * LCTR, 0);
     hipLaunchKernelGGL(k_l_fill, dim3(1), dim3(64), 0, st, (int32_t *)d.scount, (int64_t)4, 0);
@@ -995,6 +1042,10 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         const int end = std::min(R_run, r + chunk - 1);
         for (int rr = r; rr <= end; ++rr) {
             const uint32_t seed = o->seeds[rr - 1];
+            d.lact = act_list[rr & 1];
+            d.lcnt = cnt_list[rr & 1];
+            d.lprev = act_list[(rr - 1) & 1];
+            d.lcnt_prev = cnt_list[(rr - 1) & 1];
             const float *P = (const float *)g->d_params;
 #define ML_ROUND(A8_, PUSH_)                                                                                   \
     do {                                                                                                       \
@@ -1008,7 +1059,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
             else if (push) ML_ROUND(false, true);
             else ML_ROUND(false, false);
 #undef ML_ROUND
-            hipLaunchKernelGGL(k_l_reduce, dim3(1), dim3(64 * LCTR), 0, st, (const int32_t *)d.lcnt,
+            hipLaunchKernelGGL(k_l_reduce, dim3(1), dim3(256 * LCTR), 0, st, (const int32_t *)d.lcnt,
                                (const int32_t *)d.part, sgrid, d.ctr + (size_t)rr * LCTR);
         }
         ML_TRY(hipGetLastError(), "A HIP error occurred during the matching process");
