@@ -371,8 +371,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": args.copy_probe << 20, "launches": 3}), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only; multi-GPU sharding is a design note, "
-              "not a measured path. Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: the matcher runs on one GPU; sharding a graph over several GPUs is not implemented. "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     if mmamd.lib().mm_device_count() < 1:
         print("bench.py: no GPU visible to the HIP runtime", file=sys.stderr)

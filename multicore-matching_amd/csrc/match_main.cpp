@@ -1,6 +1,6 @@
 // match: command-line driver with the flags and output of the reference's
 // bin/match.cpp (flags :199-284, run loop :380-470, output :473-511), over
-// this build's drop-in classes.
+// the drop-in classes of this library.
 //
 //   match [-m "types"] [-t] [-r] [-d device] [-a repetitions] [-b barrier 0..16]
 //         [--gnuplot file] [--scaledata] [--threads "counts"] [--intermediate] graph.{graph,mtx}
