@@ -114,3 +114,12 @@ def test_generators_symmetric_sorted_mirrored_and_thread_independent(monkeypatch
 def test_rgg_mean_degree_matches_the_workload():
     csr = mmamd.gen_rgg(1 << 16, seed=12345)
     assert 10.0 < csr.nnz / csr.n < 11.0  # pi r^2 n = 0.3025 pi ln n, minus boundary loss
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 2 and "one GPU" in p.stderr and p.stdout == ""
