@@ -214,6 +214,10 @@ __device__ __forceinline__ void first_write(const Dev &d, int32_t v, int32_t vlo
 
 // G lanes per row, G entries per step; a group stops at its first candidate
 // (lowest lane of the first step that has one).  Wave-uniform control flow.
+// Software-pipelined: while a group scans its row, the list entry two steps
+// ahead and the first row chunk of the next entry are already in flight, so
+// a row that settles in its first chunk (most of them) costs one dependent
+// memory trip (the neighbour-state gathers) instead of two.
 template <bool W, bool A8, bool PUSH, bool RESPOND, int G>
 __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp,
                                   unsigned long long &nnz)
@@ -222,14 +226,24 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
     const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
     const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (gw * G));
     const int step = (int)(blockDim.x >> 6) * GPW;
-    // The next step's entry is loaded one step ahead.
+    const int4 none = make_int4(0, 0, 0, 0);
     int first = (threadIdx.x >> 6) * GPW;
-    int4 nxt = first + gw < count ? load_ent(list + first + gw) : make_int4(0, 0, 0, 0);
+    int4 cur = first + gw < count ? load_ent(list + first + gw) : none;
+    int4 nxt = first + step + gw < count ? load_ent(list + first + step + gw) : none;
+    // first chunk of cur's row
+    int32_t pu = 0;
+    float pw = 1.0f;
+    bool pok = false;
+    if (first + gw < count && cur.y + gl < cur.z) load_id<A8>(d, cur.y + gl, pu, pok, pw);
     for (; first < count; first += step) {
-        const int idx = first + gw;
-        const bool valid = idx < count;
-        const int4 ent = nxt;
-        if (first + step + gw < count) nxt = load_ent(list + first + step + gw);
+        const bool valid = first + gw < count;
+        const int4 ent = cur;
+        int4 nn = none;
+        if (first + 2 * step + gw < count) nn = load_ent(list + first + 2 * step + gw);
+        int32_t nu = 0;
+        float nw = 1.0f;
+        bool nok = false;
+        if (first + step + gw < count && nxt.y + gl < nxt.z) load_id<A8>(d, nxt.y + gl, nu, nok, nw);
         const int32_t v = ent.x;
         const int2 rg = valid ? make_int2(ent.y, ent.z) : make_int2(0, 0);
         int32_t win_u = -1;
@@ -243,7 +257,13 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
             bool cand = false;
             if (!done && j < rg.y) {
                 bool ok;
-                load_id<A8>(d, j, u, ok, w);
+                if (base == rg.x) {
+                    u = pu;
+                    ok = pok;
+                    w = pw;
+                } else {
+                    load_id<A8>(d, j, u, ok, w);
+                }
                 cand = first_candidate<W, RESPOND>(d, v, u, ok, alive);
             }
             const unsigned long long m = __ballot(cand) & gmask;
@@ -263,6 +283,11 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
             first_write<W, PUSH, RESPOND>(d, v, ent.w, win_u, win_w, alive, resp);
             nnz += (unsigned long long)(rg.y - rg.x);
         }
+        cur = nxt;
+        nxt = nn;
+        pu = nu;
+        pok = nok;
+        pw = nw;
     }
 }
 
@@ -1046,25 +1071,57 @@ int32_t analyse_csr(mm_graph *g, const int32_t *ranges, const int32_t *nbrs, con
 // order: neighbours get nearby positions, so the per-round gathers of vertex
 // state hit the same cache lines.  order[new] = old.  Components never mix,
 // so every batched graph keeps one contiguous block of new ids, in order.
-std::vector<int32_t> bfs_order(int32_t n, const int32_t *ranges, const int32_t *nbrs)
+std::vector<int32_t> bfs_order(int32_t n, const int32_t *ranges, const int32_t *nbrs, const int32_t *vbase)
 {
-    std::vector<int32_t> order((size_t)n);
+    // Breadth-first over the vertices with neighbours (components in order of
+    // their smallest id); vertices without neighbours are then spread evenly
+    // over the order instead of trailing it.  Buckets are ranges of the order,
+    // and an isolated vertex costs a bucket almost nothing, so a graph with
+    // many of them (R-MAT: about half) would otherwise leave the giant
+    // component's buckets with all of the work.  A batch (vbase: first id of
+    // each vertex's graph) is ordered graph by graph, so every graph keeps a
+    // contiguous range of the order.
+    std::vector<int32_t> order((size_t)n), conn, iso;
     std::vector<uint8_t> seen((size_t)n, 0);
-    size_t head = 0, tail = 0;
-    for (int32_t s0 = 0; s0 < n; ++s0) {
-        if (seen[(size_t)s0]) continue;
-        seen[(size_t)s0] = 1;
-        order[tail++] = s0;
-        while (head < tail) {
-            const int32_t v = order[head++];
-            for (int32_t j = ranges[2 * (size_t)v]; j < ranges[2 * (size_t)v + 1]; ++j) {
-                const int32_t u = nbrs[j];
-                if (!seen[(size_t)u]) {
-                    seen[(size_t)u] = 1;
-                    order[tail++] = u;
+    int32_t lo = 0;
+    while (lo < n) {
+        int32_t hi = lo + 1;
+        if (vbase)
+            while (hi < n && vbase[(size_t)hi] != hi) ++hi;
+        else
+            hi = n;
+        conn.clear();
+        iso.clear();
+        size_t head = 0;
+        for (int32_t s0 = lo; s0 < hi; ++s0) {
+            if (seen[(size_t)s0]) continue;
+            seen[(size_t)s0] = 1;
+            if (ranges[2 * (size_t)s0 + 1] == ranges[2 * (size_t)s0]) {
+                iso.push_back(s0);  // (a row naming it from elsewhere will not add it again)
+                continue;
+            }
+            conn.push_back(s0);
+            while (head < conn.size()) {
+                const int32_t v = conn[head++];
+                for (int32_t j = ranges[2 * (size_t)v]; j < ranges[2 * (size_t)v + 1]; ++j) {
+                    const int32_t u = nbrs[j];
+                    if (!seen[(size_t)u]) {
+                        seen[(size_t)u] = 1;
+                        conn.push_back(u);
+                    }
                 }
             }
         }
+        // [lo, hi) takes this segment's vertices: conn and iso hold hi - lo ids
+        // (a batch's rows stay inside their graph).
+        const int64_t len = hi - lo, k = (int64_t)iso.size();
+        size_t ci = 0, ii = 0;
+        for (int64_t p = 0; p < len; ++p) {
+            // position p holds an isolated vertex when floor((p + 1) k / len) steps up
+            const bool take_iso = ((p + 1) * k) / len > (p * k) / len;
+            order[(size_t)(lo + p)] = take_iso ? iso[ii++] : conn[ci++];
+        }
+        lo = hi;
     }
     return order;
 }
@@ -1142,7 +1199,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     //     candidate in the row.
     g->relabelled = relabel != 0 && n > 1;
     std::vector<int32_t> order;
-    if (g->relabelled) order = bfs_order(n, ranges, nbrs);
+    if (g->relabelled) order = bfs_order(n, ranges, nbrs, vbase_old);
     std::vector<int32_t> inv;
     if (g->relabelled) {
         inv.resize(nn);
