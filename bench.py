@@ -171,7 +171,7 @@ def timed(fn, warmup: int, steps: int) -> float:
 def run_single(args, csr, name: str, workload_desc: str) -> dict:
     n = csr.n
     t0 = time.perf_counter()
-    g = mmamd.Graph(csr, keep_order=args.keep_order, packed_rows=args.packed_rows)
+    g = mmamd.Graph(csr, keep_order=args.keep_order, packed_rows=args.packed_rows, heads=args.heads)
     log(f"graph upload (incl. renumbering) {time.perf_counter() - t0:.2f} s")
     seeds = mmamd.reference_seeds(mmamd.MM_MAX_ROUNDS)
     base = dict(respond=args.respond)
@@ -225,7 +225,7 @@ def run_single(args, csr, name: str, workload_desc: str) -> dict:
     achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
     log(f"profile: kernel_ms={kms} launches={prof.kernel_launches} bytes={kb}")
     traffic = None
-    if not args.keep_order and not args.packed_rows and args.respond == 0:
+    if not args.keep_order and not args.packed_rows and args.heads and args.respond == 0:
         traffic = measured_traffic(f"{name}_weighted_t99", dom, tiles)
 
     line = {
@@ -357,6 +357,8 @@ def main():
                     help="network parameters of the learned selection (learn.py train)")
     ap.add_argument("--keep-order", action="store_true", help="no internal breadth-first renumbering")
     ap.add_argument("--packed-rows", action="store_true", help="device rows back to back (no line alignment)")
+    ap.add_argument("--no-heads", dest="heads", action="store_false",
+                    help="no dense copy of the rows' first entries (round 1 scans the rows)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="run only a plain device copy of this many MiB (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()

@@ -275,6 +275,15 @@ int32_t run_check(mm_graph *g, const int32_t *voff, int32_t n_graphs, mm_check *
             for (void *q : ptrs) (void)hipFree(q);
         }
     } tmp;
+    if (g->internal_stale) {
+        // The last plain run wrote only the caller-order result: bring the
+        // internal-order array the check reads up to date.
+        const int pgrid = (int)std::min<int64_t>(((int64_t)g->n + 255) / 256, (int64_t)g->num_cu * 8);
+        hipLaunchKernelGGL(k_c_permute, dim3(pgrid), dim3(256), 0, g->stream, (const int2 *)g->d_unperm, g->n,
+                           (const int32_t *)g->d_match_out, g->d_match);
+        CK_TRY(hipGetLastError(), "k_c_permute");
+        g->internal_stale = false;
+    }
     CDev d;
     std::memset(&d, 0, sizeof(d));
     d.n = g->n;
@@ -370,6 +379,7 @@ int32_t mm_graph_set_match(mm_graph *g, const int32_t *match)
         CK_TRY(hipMemcpy(g->d_match, match, bytes, hipMemcpyHostToDevice), "Unable to set match");
         return MM_OK;
     }
+    g->internal_stale = false;
     CK_TRY(hipMemcpy(g->d_match_out, match, bytes, hipMemcpyHostToDevice), "Unable to set match");
     const int grid = (int)std::min<int64_t>(((int64_t)g->n + 255) / 256, (int64_t)g->num_cu * 8);
     hipLaunchKernelGGL(k_c_permute, dim3(grid), dim3(256), 0, g->stream, (const int2 *)g->d_unperm, g->n,

@@ -64,7 +64,9 @@ struct Dev {
     const int32_t *nbrs;  // unweighted layout
     const int2 *adj;      // weighted rows sorted for matching: {neighbour id (~id when the
                           // weight can never win), weight bits}
-    int32_t *match;
+    const int2 *head;     // entries 0..3 of every adj row, 4 per vertex (null: none kept)
+    int32_t *out;         // result in the caller's order, written as vertices settle
+    const int32_t *vbase; // batches: first caller id of each vertex's graph (null: single graph)
     int32_t *req;
     uint8_t *st;
     unsigned long long *resp;  // [2][n], PUSH only
@@ -149,6 +151,51 @@ __device__ __forceinline__ void load_id(const Dev &d, int32_t j, int32_t &u, boo
     }
 }
 
+// Raw row entry j ({id, weight bits} weighted, {id, 0} otherwise), decoded
+// only where it is used: a load whose value is not read yet costs no wait.
+template <bool A8>
+__device__ __forceinline__ int2 load_raw(const Dev &d, int32_t j)
+{
+    typedef int v2i __attribute__((ext_vector_type(2)));
+    if (A8) {
+        const v2i e = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(d.adj + j));
+        return make_int2(e.x, e.y);
+    }
+    return make_int2(__builtin_nontemporal_load(d.nbrs + j), 0);
+}
+
+template <bool A8>
+__device__ __forceinline__ void decode_raw(int2 e, int32_t &u, bool &valid, float &w)
+{
+    if (A8) {
+        valid = e.x >= 0;
+        u = valid ? e.x : ~e.x;
+        w = valid ? __int_as_float(e.y) : -2.0f;
+    } else {
+        u = e.x;
+        valid = true;
+        w = 1.0f;
+    }
+}
+
+// First row step of list entry e for lane gl: entry e.y + gl of the row, from
+// the dense head copy when HEAD (entries 0..3 of v = e.x's row, 4 per vertex).
+// Always a valid address -- clamped into the row, or entry 0 for an empty row
+// (the caller masks lanes past the row end) -- so the load can be issued
+// unconditionally: a load under a branch makes the compiler wait for every
+// load in flight at the first use after it.
+template <bool A8, bool HEAD>
+__device__ __forceinline__ int2 first_raw(const Dev &d, const int4 &e, int gl)
+{
+    typedef int v2i __attribute__((ext_vector_type(2)));
+    if (HEAD) {
+        const v2i h = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(d.head + (size_t)e.x * 4 + gl));
+        return make_int2(h.x, h.y);
+    }
+    const int32_t j = e.y + gl < e.z ? e.y + gl : (e.z > e.y ? e.z - 1 : 0);
+    return load_raw<A8>(d, j);
+}
+
 template <bool A8>
 __device__ __forceinline__ void load_entry(const Dev &d, int32_t j, int32_t &u, float &w)
 {
@@ -176,6 +223,22 @@ __device__ __forceinline__ bool first_candidate(const Dev &d, int32_t v, int32_t
     bool cand;
     if (RESPOND) {
         cand = s == ST_BLUE && d.req[u] == v;
+    } else {
+        alive = alive || s != ST_MATCHED;
+        cand = s == ST_RED;
+    }
+    if (W) cand = cand && valid;
+    return cand;
+}
+
+// first_candidate on an already gathered state s = st[u] (and, RESPOND,
+// q = req[u]).
+template <bool W, bool RESPOND>
+__device__ __forceinline__ bool candidate_of(int32_t v, uint8_t s, int32_t q, bool valid, bool &alive)
+{
+    bool cand;
+    if (RESPOND) {
+        cand = s == ST_BLUE && q == v;
     } else {
         alive = alive || s != ST_MATCHED;
         cand = s == ST_RED;
@@ -217,77 +280,143 @@ __device__ __forceinline__ void first_write(const Dev &d, int32_t v, int32_t vlo
 // Software-pipelined: while a group scans its row, the list entry two steps
 // ahead and the first row chunk of the next entry are already in flight, so
 // a row that settles in its first chunk (most of them) costs one dependent
-// memory trip (the neighbour-state gathers) instead of two.
-template <bool W, bool A8, bool PUSH, bool RESPOND, int G>
+// memory trip (the neighbour-state gathers) instead of two.  Vector loads
+// return in issue order, so the gathers of the first step are issued before
+// the prefetches: waiting for them then leaves the prefetches (HBM) in flight
+// instead of waiting for those too.  HEAD (G == 4):
+// the first step reads the row's 4 leading entries from the dense head copy
+// (32 bytes per vertex, so the rows of one wave share lines) instead of the
+// row's own line.
+// Steps 1, 2, ... of a row scan (entries from base on), G lanes per row; the
+// group stops at its first candidate.  Wave-uniform control flow: every lane
+// of the wavefront must call it (done lanes idle).
+template <bool W, bool A8, bool RESPOND, int G>
+__device__ __forceinline__ void scan_rest(const Dev &d, int32_t v, int2 rg, int32_t base, bool done, bool &alive,
+                                          int32_t &win_u, float &win_w)
+{
+    const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
+    const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (gw * G));
+    if (base >= rg.y) done = true;
+    while (__ballot(!done) != 0ull) {
+        const int32_t j = base + gl;
+        int32_t u = 0;
+        float w = 1.0f;
+        bool cand = false;
+        if (!done && j < rg.y) {
+            bool ok;
+            load_id<A8>(d, j, u, ok, w);
+            const uint8_t su8 = d.st[u];
+            const int32_t sq = RESPOND ? d.req[u] : 0;
+            cand = candidate_of<W, RESPOND>(v, su8, sq, ok, alive);
+        }
+        const unsigned long long m = __ballot(cand) & gmask;
+        const int src = m ? __ffsll((long long)m) - 1 : lane;
+        const int32_t su = __shfl(u, src);
+        const float sw = __shfl(w, src);
+        if (!done && m) {
+            win_u = su;
+            win_w = sw;
+            done = true;
+        }
+        base += G;
+        if (base >= rg.y) done = true;
+    }
+}
+
+// Rows deferred to a block's second pass: a first step that settles no row
+// keeps the whole wavefront waiting for the next dependent step, so rows that
+// need more than one step are set aside (in dynamic LDS, one 16-byte entry
+// per thread of the block) and scanned together after the first steps of all
+// rows; a row that finds no room is finished in place.
+extern __shared__ __align__(16) int4 s_deferred[];
+
+template <bool W, bool A8, bool PUSH, bool RESPOND, int G, bool HEAD = false>
 __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp,
                                   unsigned long long &nnz)
 {
+    static_assert(!HEAD || (G == 4 && A8), "head steps are 4 sorted entries");
     constexpr int GPW = 64 / G;
+    int4 *s_def = s_deferred;
+    const int DEF_CAP = (int)blockDim.x;
+    __shared__ int32_t s_ndef;
+    if (count <= 0) return;
+    if (threadIdx.x == 0) s_ndef = 0;
+    __syncthreads();
     const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
     const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (gw * G));
     const int step = (int)(blockDim.x >> 6) * GPW;
     const int4 none = make_int4(0, 0, 0, 0);
+    // list entries are read at clamped indices (always in bounds); validity
+    // is decided by the unclamped index
+    auto ent_at = [&](int32_t i) { return load_ent(list + min(i, count - 1)); };
     int first = (threadIdx.x >> 6) * GPW;
-    int4 cur = first + gw < count ? load_ent(list + first + gw) : none;
-    int4 nxt = first + step + gw < count ? load_ent(list + first + step + gw) : none;
-    // first chunk of cur's row
-    int32_t pu = 0;
-    float pw = 1.0f;
-    bool pok = false;
-    if (first + gw < count && cur.y + gl < cur.z) load_id<A8>(d, cur.y + gl, pu, pok, pw);
+    int4 cur = ent_at(first + gw);
+    int4 nxt = ent_at(first + step + gw);
+    int2 praw = first_raw<A8, HEAD>(d, cur, gl);  // first step of cur's row
     for (; first < count; first += step) {
         const bool valid = first + gw < count;
-        const int4 ent = cur;
-        int4 nn = none;
-        if (first + 2 * step + gw < count) nn = load_ent(list + first + 2 * step + gw);
-        int32_t nu = 0;
-        float nw = 1.0f;
-        bool nok = false;
-        if (first + step + gw < count && nxt.y + gl < nxt.z) load_id<A8>(d, nxt.y + gl, nu, nok, nw);
-        const int32_t v = ent.x;
-        const int2 rg = valid ? make_int2(ent.y, ent.z) : make_int2(0, 0);
+        const int4 e = cur;
+        const int32_t v = e.x;
+        const int2 rg = valid ? make_int2(e.y, e.z) : make_int2(0, 0);
+        int32_t pu;
+        bool pok;
+        float pw;
+        decode_raw<A8>(praw, pu, pok, pw);
+        // (lanes past the row end gather vertex 0's slot: an edgeless graph's
+        // row storage holds no ids at all)
+        const bool in0 = rg.x + gl < rg.y;
+        const int32_t g0 = in0 ? pu : 0;
+        const uint8_t s0 = d.st[g0];
+        const int32_t q0 = RESPOND ? d.req[g0] : 0;
+        // keep the scheduler from hoisting the prefetches above the gathers
+        __builtin_amdgcn_sched_barrier(0);
+        const int4 nn = ent_at(first + 2 * step + gw);
+        const int2 nraw = first_raw<A8, HEAD>(d, nxt, gl);
+        // step 0
         int32_t win_u = -1;
         float win_w = 1.0f;
         bool alive = false, done = rg.x >= rg.y;
-        int32_t base = rg.x;
-        while (__ballot(!done) != 0ull) {
-            const int32_t j = base + gl;
-            int32_t u = 0;
-            float w = 1.0f;
-            bool cand = false;
-            if (!done && j < rg.y) {
-                bool ok;
-                if (base == rg.x) {
-                    u = pu;
-                    ok = pok;
-                    w = pw;
-                } else {
-                    load_id<A8>(d, j, u, ok, w);
-                }
-                cand = first_candidate<W, RESPOND>(d, v, u, ok, alive);
-            }
-            const unsigned long long m = __ballot(cand) & gmask;
-            const int src = m ? __ffsll((long long)m) - 1 : lane;
-            const int32_t su = __shfl(u, src);
-            const float sw = __shfl(w, src);
-            if (!done && m) {
-                win_u = su;
-                win_w = sw;
-                done = true;
-            }
-            base += G;
-            if (base >= rg.y) done = true;
+        const bool cand = !done && in0 && candidate_of<W, RESPOND>(v, s0, q0, pok, alive);
+        const unsigned long long m = __ballot(cand) & gmask;
+        const int src = m ? __ffsll((long long)m) - 1 : lane;
+        const int32_t su = __shfl(pu, src);
+        const float sw = __shfl(pw, src);
+        if (!done && m) {
+            win_u = su;
+            win_w = sw;
+            done = true;
         }
+        if (rg.x + G >= rg.y) done = true;
+        alive = (__ballot(alive) & gmask) != 0ull;
+        // unsettled rows go to the second pass while LDS room lasts
+        int slot = -1;
+        if (!done && gl == 0) slot = atomicAdd(&s_ndef, 1);
+        slot = __shfl(slot, lane & ~(G - 1));
+        const bool deferred = slot >= 0 && slot < DEF_CAP;
+        if (deferred && gl == 0) s_def[slot] = make_int4(v | (alive ? INT_MIN : 0), rg.x + G, rg.y, e.w);
+        scan_rest<W, A8, RESPOND, G>(d, v, rg, rg.x + G, done || deferred, alive, win_u, win_w);
         alive = (__ballot(alive) & gmask) != 0ull;
         if (valid && gl == 0) {
-            first_write<W, PUSH, RESPOND>(d, v, ent.w, win_u, win_w, alive, resp);
+            if (!deferred) first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp);
             nnz += (unsigned long long)(rg.y - rg.x);
         }
         cur = nxt;
         nxt = nn;
-        pu = nu;
-        pok = nok;
-        pw = nw;
+        praw = nraw;
+    }
+    __syncthreads();
+    const int nd = min(s_ndef, DEF_CAP);
+    for (int i0 = (threadIdx.x >> 6) * GPW; i0 < nd; i0 += step) {
+        const bool valid = i0 + gw < nd;
+        const int4 e = valid ? s_def[i0 + gw] : none;
+        const int32_t v = e.x & INT_MAX;
+        bool alive = e.x < 0;
+        const int2 rg = make_int2(e.y, e.z);
+        int32_t win_u = -1;
+        float win_w = 1.0f;
+        scan_rest<W, A8, RESPOND, G>(d, v, rg, rg.x, !valid, alive, win_u, win_w);
+        alive = (__ballot(alive) & gmask) != 0ull;
+        if (valid && gl == 0) first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp);
     }
 }
 
@@ -400,29 +529,42 @@ __global__ void k_init_select(Dev d)
     int4 *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
     int4 *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
     unsigned long long deg_sum = 0, hub_deg_sum = 0;
-    for (int64_t base = lo + (threadIdx.x & ~63); base < hi; base += blockDim.x) {
-        const int32_t v = (int32_t)(base + (threadIdx.x & 63));
-        const bool valid = v < hi;
-        int cls = -1;
-        int4 ent = make_int4(v, 0, 0, 0);
-        if (valid) {
-            const int32_t vloc = d.loc[v];
-            const int c = colour_of((uint32_t)vloc, seed, d.barrier);
-            d.match[v] = c;
-            d.st[v] = (uint8_t)c;
-            if (d.vround) d.vround[v] = 255;
-            const int2 rg = d.ranges[v];
-            ent = make_int4(v, rg.x, rg.y, vloc);
-            const int hub = (rg.y - rg.x) > d.hub_deg ? 1 : 0;
-            cls = 2 * c + hub;
-            if (hub) hub_deg_sum += (unsigned long long)(rg.y - rg.x);
-            else deg_sum += (unsigned long long)(rg.y - rg.x);
-            if (PUSH && c == ST_RED) d.resp[(size_t)parity * d.n + v] = 0ull;
+    // IU vertices per thread and step, all loads issued before any hash.
+    constexpr int IU = 4;
+    for (int64_t base = lo + (threadIdx.x & ~63); base < hi; base += (int64_t)blockDim.x * IU) {
+        int32_t vl[IU];
+        int2 rg[IU];
+#pragma unroll
+        for (int q = 0; q < IU; ++q) {
+            const int64_t v = base + (int64_t)q * blockDim.x + (threadIdx.x & 63);
+            vl[q] = 0;
+            rg[q] = make_int2(0, 0);
+            if (v < hi) {
+                vl[q] = d.loc[v];
+                rg[q] = d.ranges[v];
+            }
         }
-        bucket_append(cls == L_BLUE_LO, ent, blue, s_cnt + 0);
-        bucket_append(cls == L_RED_LO, ent, red, s_cnt + 1);
-        wave_append(cls == L_BLUE_HI, ent, list_ptr(d, parity, L_BLUE_HI), d.ctr + 1 * CTR_STRIDE + L_BLUE_HI);
-        wave_append(cls == L_RED_HI, ent, list_ptr(d, parity, L_RED_HI), d.ctr + 1 * CTR_STRIDE + L_RED_HI);
+#pragma unroll
+        for (int q = 0; q < IU; ++q) {
+            const int32_t v = (int32_t)(base + (int64_t)q * blockDim.x + (threadIdx.x & 63));
+            int cls = -1;
+            const int4 ent = make_int4(v, rg[q].x, rg[q].y, vl[q]);
+            if (v < hi) {
+                const int c = colour_of((uint32_t)vl[q], seed, d.barrier);
+                d.st[v] = (uint8_t)c;
+                if (d.vround) d.vround[v] = 255;
+                const int32_t deg = rg[q].y - rg[q].x;
+                const int hub = deg > d.hub_deg ? 1 : 0;
+                cls = 2 * c + hub;
+                if (hub) hub_deg_sum += (unsigned long long)deg;
+                else deg_sum += (unsigned long long)deg;
+                if (PUSH && c == ST_RED) d.resp[(size_t)parity * d.n + v] = 0ull;
+            }
+            bucket_append(cls == L_BLUE_LO, ent, blue, s_cnt + 0);
+            bucket_append(cls == L_RED_LO, ent, red, s_cnt + 1);
+            wave_append(cls == L_BLUE_HI, ent, list_ptr(d, parity, L_BLUE_HI), d.ctr + 1 * CTR_STRIDE + L_BLUE_HI);
+            wave_append(cls == L_RED_HI, ent, list_ptr(d, parity, L_RED_HI), d.ctr + 1 * CTR_STRIDE + L_RED_HI);
+        }
     }
     if (d.count_nnz) {
         deg_sum = block_sum(deg_sum);
@@ -442,7 +584,7 @@ __global__ void k_init_select(Dev d)
 // Request phase of round r.  Blocks [0, nb) take bucket b of the blue rows,
 // G lanes per row; blocks [nb, nb + hub_blocks) take the hub rows, one block
 // per row.
-template <bool W, bool A8, bool PUSH, int G>
+template <bool W, bool A8, bool PUSH, int G, bool HEAD = false>
 __global__ void k_request(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
@@ -455,8 +597,8 @@ __global__ void k_request(Dev d, int r, int hub_blocks)
         return;
     }
     const int b = bucket_of_block(d);
-    first_scan_bucket<W, A8, PUSH, false, G>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap,
-                                             ctrb_ptr(d, r, 0)[b], resp, nnz);
+    first_scan_bucket<W, A8, PUSH, false, G, HEAD>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap,
+                                                   ctrb_ptr(d, r, 0)[b], resp, nnz);
     if (d.count_nnz) {
         nnz = block_sum(nnz);
         if (threadIdx.x == 0) ctrb64_ptr(d, r, 1)[b] = nnz;
@@ -672,29 +814,45 @@ __global__ void k_respond(Dev d, int r, int hub_blocks)
 // and, when do_select, its colour for round r+1 (gSelect).  Returns the class
 // of v in round r+1, or -1 when it left the active set.
 // partner is the partner's graph-local original id (-1: none); vloc is v's.
+// Final values go straight into the caller-order result as vertices settle
+// (a 4-byte scattered store per vertex over the whole run, instead of a
+// separate permutation pass at its end).  out index of a vertex with
+// graph-local id loc in the graph of internal vertex v: vbase[v] + loc.
+__device__ __forceinline__ int32_t out_base(const Dev &d, int32_t v) { return d.vbase ? d.vbase[v] : 0; }
+
+// partner_of's value for a blue vertex of a PUSH round that won its proposal:
+// its result is written by the red end, which knows both ids.
+constexpr int32_t PARTNER_WRITES = INT_MAX;
+
 template <bool PUSH>
 __device__ __forceinline__ int settle_vertex(const Dev &d, int32_t v, int32_t vloc, int colour, int hub,
                                              int32_t partner, bool dead, int r, int do_select, uint32_t seed,
                                              unsigned long long *nresp)
 {
     if (partner >= 0) {
-        d.match[v] = 4 + min(vloc, partner);
+        if (partner != PARTNER_WRITES) {
+            const int32_t m = 4 + min(vloc, partner), ob = out_base(d, v);
+            d.out[ob + vloc] = m;
+            // PUSH: the red end also writes the blue end's value (same graph)
+            if (PUSH && colour == ST_RED) d.out[ob + partner] = m;
+        }
         d.st[v] = ST_MATCHED;
         if (d.vround) d.vround[v] = (uint8_t)r;
         return -1;
     }
     if (dead) {
-        d.match[v] = 2;
+        d.out[out_base(d, v) + vloc] = 2;
         d.st[v] = ST_DEAD;
         if (d.vround) d.vround[v] = (uint8_t)r;
         return -1;
     }
-    if (!do_select) return -1;
-    const int c = colour_of((uint32_t)vloc, seed, d.barrier);
-    if (c != colour) {
-        d.match[v] = c;
-        d.st[v] = (uint8_t)c;
+    if (!do_select) {
+        // the run ends with v still active: its value is this round's colour
+        d.out[out_base(d, v) + vloc] = colour;
+        return -1;
     }
+    const int c = colour_of((uint32_t)vloc, seed, d.barrier);
+    if (c != colour) d.st[v] = (uint8_t)c;
     if (PUSH && c == ST_RED) nresp[v] = 0ull;
     return 2 * c + hub;
 }
@@ -702,7 +860,8 @@ __device__ __forceinline__ int settle_vertex(const Dev &d, int32_t v, int32_t vl
 // Partner of v after round r's responses, as a graph-local original id, from
 // its request a = req[v], the word k read for it (PUSH: the proposal key at the
 // red end, whose low half is the winner's local id; PULL: req[] of the other
-// end) and pl = loc[a].  -1 when unmatched; dead flags a blue vertex without
+// end) and pl = loc[a] (PULL only).  -1 when unmatched; PARTNER_WRITES for a
+// blue vertex whose proposal won (PUSH); dead flags a blue vertex without
 // unmatched neighbours.
 template <bool PUSH>
 __device__ __forceinline__ int32_t partner_of(int32_t v, int32_t vloc, int colour, int32_t a, unsigned long long k,
@@ -714,7 +873,7 @@ __device__ __forceinline__ int32_t partner_of(int32_t v, int32_t vloc, int colou
             return -1;
         }
         if (a < 0) return -1;
-        if (PUSH) return (uint32_t)k == 0xFFFFFFFFu - (uint32_t)vloc ? pl : -1;
+        if (PUSH) return (uint32_t)k == 0xFFFFFFFFu - (uint32_t)vloc ? PARTNER_WRITES : -1;
         return (int32_t)k == v ? pl : -1;
     }
     if (PUSH) return k != 0ull ? (int32_t)(0xFFFFFFFFu - (uint32_t)k) : -1;
@@ -741,8 +900,8 @@ __device__ __forceinline__ void partner_words2(const Dev &d, int colour, const u
         if (colour == ST_BLUE) k = resp[a];
     } else {
         k = (unsigned long long)(uint32_t)d.req[a];
+        pl = d.loc[a];
     }
-    pl = d.loc[a];
 }
 
 // Match write-back of round r fused with the select of round r+1: block b
@@ -976,7 +1135,7 @@ void release(mm_graph *g)
     if (!g) return;
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
-    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_match, g->d_req,   g->d_st,
+    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_head,  g->d_match, g->d_req,   g->d_st,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
                     g->d_unperm, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_lcnt, g->d_llist, g->d_params, g->d_vbase, g->d_chk, g->d_cpart,
@@ -1129,7 +1288,8 @@ std::vector<int32_t> bfs_order(int32_t n, const int32_t *ranges, const int32_t *
 // vbase_old: per ORIGINAL vertex, the first id of its batched graph (null for
 // a single graph).  relabel: renumber the vertices internally (bfs_order).
 int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *nbrs,
-                     const float *w, const int32_t *vbase_old, int relabel, bool align_rows, mm_graph **out)
+                     const float *w, const int32_t *vbase_old, int relabel, bool align_rows, bool heads,
+                     mm_graph **out)
 {
     mm::clear_error();
     if (!out) return mm::set_error(MM_ERR_INVALID, "mm_graph_create: out is NULL");
@@ -1270,6 +1430,20 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     if (n > 0 && (hipMemcpy(g->d_ranges, h_ranges.data(), sizeof(int2) * nn, hipMemcpyHostToDevice) != hipSuccess ||
                   hipMemcpy(g->d_loc, h_loc.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess))
         return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+    if (g->weighted && heads && n > 0) {
+        // Dense copy of each sorted row's first 4 entries (the gap of a shorter
+        // row is never read: scans stop at the row end).
+        std::vector<int2> h_head(4 * nn, make_int2(0, 0));
+        mm::parallel_for_dynamic(n, 4096, [&](int64_t b, int64_t e, unsigned) {
+            for (int64_t i = b; i < e; ++i) {
+                const int32_t lo = h_ranges[2 * (size_t)i], hi = h_ranges[2 * (size_t)i + 1];
+                for (int32_t k = 0; k < 4 && lo + k < hi; ++k) h_head[4 * (size_t)i + k] = h_adj[(size_t)lo + k];
+            }
+        });
+        if ((rc = dev_alloc(g, &g->d_head, 4 * nn, "row heads")) != MM_OK) return fail(rc);
+        if (hipMemcpy(g->d_head, h_head.data(), sizeof(int2) * 4 * nn, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+    }
     if (nnz_dev > 0) {
         if (hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz_dev, hipMemcpyHostToDevice) !=
                 hipSuccess ||
@@ -1324,23 +1498,27 @@ int64_t hubs_above(const mm_graph *g, int32_t thr)
     return s;
 }
 
+// Dynamic LDS of the first-candidate scan kernels: the deferred-row buffer.
+size_t deferred_smem_bytes(int block) { return sizeof(int4) * (size_t)block; }
+
 struct RunCfg {
     bool W, A8, PUSH;
+    bool HEAD;  // round 1 scans the head copy, 4 lanes per row
     int G, U, block, hub_deg, nb, bcap, hub_blocks;
 };
 
 template <bool W, bool A8, bool PUSH, int G>
 void launch_request_t(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_request<W, A8, PUSH, G>), dim3(c.nb + c.hub_blocks), dim3(c.block), 0, s, d, r,
-                       c.hub_blocks);
+    hipLaunchKernelGGL((k_request<W, A8, PUSH, G>), dim3(c.nb + c.hub_blocks), dim3(c.block),
+                       deferred_smem_bytes(c.block), s, d, r, c.hub_blocks);
 }
 
 template <bool W, bool A8, int G>
 void launch_respond_t(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_respond<W, A8, G>), dim3(c.nb + c.hub_blocks), dim3(c.block), 0, s, d, r,
-                       c.hub_blocks);
+    hipLaunchKernelGGL((k_respond<W, A8, G>), dim3(c.nb + c.hub_blocks), dim3(c.block),
+                       deferred_smem_bytes(c.block), s, d, r, c.hub_blocks);
 }
 
 template <bool W, bool A8, bool PUSH>
@@ -1383,6 +1561,13 @@ void launch_request_tile(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 
 void launch_request(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
+    if (r == 1 && c.HEAD) {
+        const dim3 grid(c.nb + c.hub_blocks), blk(c.block);
+        const size_t sm = deferred_smem_bytes(c.block);
+        if (c.PUSH) hipLaunchKernelGGL((k_request<true, true, true, 4, true>), grid, blk, sm, s, d, r, c.hub_blocks);
+        else hipLaunchKernelGGL((k_request<true, true, false, 4, true>), grid, blk, sm, s, d, r, c.hub_blocks);
+        return;
+    }
     if (c.G == 0) {
         if (c.W) {
             if (c.A8) { if (c.PUSH) launch_request_tile<true, true, true>(c, d, r, s); else launch_request_tile<true, true, false>(c, d, r, s); }
@@ -1446,6 +1631,8 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     if (o->respond < 0 || o->respond > MM_RESPOND_PUSH) return mm::set_error(MM_ERR_INVALID, "mm_match: bad respond mode");
     c.PUSH = o->respond == MM_RESPOND_PUSH || (o->respond == MM_RESPOND_AUTO && push_ok);
     c.G = G;
+    // The head copy serves the first round of weighted scans (not the tiles).
+    c.HEAD = c.A8 && g->d_head != nullptr && G != 0;
     c.U = o->entries_per_thread > 0 ? o->entries_per_thread : 4;
     if (c.U != 4 && c.U != 8 && c.U != 16)
         return mm::set_error(MM_ERR_INVALID, "mm_match: entries_per_thread must be 4, 8 or 16");
@@ -1481,7 +1668,11 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.ranges = g->d_ranges;
     d.nbrs = g->d_nbrs;
     d.adj = g->d_adj;
-    d.match = g->d_match;
+    d.head = c.HEAD ? g->d_head : nullptr;
+    // Vertices write their final values straight into the caller-order result
+    // as they settle, so no permutation pass follows the last round.
+    d.out = g->relabelled ? g->d_match_out : g->d_match;
+    d.vbase = g->d_vbase;
     d.req = g->d_req;
     d.st = g->d_st;
     d.resp = g->d_resp;
@@ -1570,8 +1761,6 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
             else MM_LAUNCH(K_MATCH, hipLaunchKernelGGL(k_match_select<false>, dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
         }
         if (maximal) return enq_reduce(r0, std::min(r1 + 1, R_run));
-        // A fixed run ends with the result in the caller's order, in the same sequence.
-        if (r1 == R_run) mm_graph_enqueue_unpermute(g);
         return MM_OK;
     };
     // Replays the captured launch sequence of rounds [r0, r1], capturing it on first use.
@@ -1642,9 +1831,9 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
         return mm::set_error(MM_ERR_SEEDS, "mm_match: ran out of seeds before the matching converged");
     if (g->n == 0) rounds = maximal ? 1 : R_max;
     if (rounds < 0) rounds = R_max;
-    // The result in the caller's numbering is part of the matching (fixed runs
-    // enqueue it with their last round).
-    if (maximal) mm_graph_enqueue_unpermute(g);
+    // The result is already in the caller's numbering; the internal-order
+    // array is rebuilt from it only if a device check asks for it.
+    g->internal_stale = g->relabelled;
     MM_HIP_TRY(hipEventRecord(g->ev_t2, g->stream), "hipEventRecord");
     // Totals of every executed round (a maximal run's last, empty round has none).
     const int r_stats = g->n > 0 ? std::min(rounds, R_run) : 0;
@@ -1855,16 +2044,16 @@ int32_t mm_device_count(void)
 int32_t mm_graph_create(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *neighbours,
                         const float *weights, mm_graph **out)
 {
-    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, 1, true, out);
+    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, 1, true, true, out);
 }
 
 int32_t mm_graph_create_ex(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *neighbours,
                            const float *weights, uint32_t flags, mm_graph **out)
 {
-    if (flags & ~(uint32_t)(MM_CREATE_KEEP_ORDER | MM_CREATE_PACKED_ROWS))
+    if (flags & ~(uint32_t)(MM_CREATE_KEEP_ORDER | MM_CREATE_PACKED_ROWS | MM_CREATE_NO_HEADS))
         return mm::set_error(MM_ERR_INVALID, "mm_graph_create_ex: bad flags");
     return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, (flags & MM_CREATE_KEEP_ORDER) ? 0 : 1,
-                        (flags & MM_CREATE_PACKED_ROWS) == 0, out);
+                        (flags & MM_CREATE_PACKED_ROWS) == 0, (flags & MM_CREATE_NO_HEADS) == 0, out);
 }
 
 int32_t mm_graph_destroy(mm_graph *g)
@@ -1947,7 +2136,7 @@ int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_
     if (bad) return mm::set_error(MM_ERR_INVALID, "mm_batch_create: malformed per-graph CSR");
     mm_graph *g = nullptr;
     int32_t rc = create_graph(device, (int32_t)total_n, total_nnz, granges.data(), gnbrs.data(), weights,
-                              vbase.data(), 1, true, &g);
+                              vbase.data(), 1, true, true, &g);
     if (rc != MM_OK) return rc;
     g->n_graphs = n_graphs;
     if ((rc = dev_alloc(g, &g->d_voff, (size_t)n_graphs + 1, "batch offsets")) != MM_OK ||

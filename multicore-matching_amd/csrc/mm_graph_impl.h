@@ -35,6 +35,7 @@ struct mm_graph {
     int2 *d_ranges = nullptr;
     int32_t *d_nbrs = nullptr;
     int2 *d_adj = nullptr;
+    int2 *d_head = nullptr;   // weighted: entries 0..3 of each sorted adj row, 4 per vertex (round-1 scans)
     int32_t *d_match = nullptr;
     int32_t *d_req = nullptr;
     uint8_t *d_st = nullptr;
@@ -60,6 +61,7 @@ struct mm_graph {
     int32_t *d_chk = nullptr;        // [3][n] scratch of the device result check
     unsigned long long *d_cpart = nullptr;  // per-block partial sums of the device result check
     bool relabelled = false;
+    bool internal_stale = false;     // d_match is behind d_match_out (the plain rounds write only the latter)
     int32_t *d_voff = nullptr;
     uint8_t *d_vround = nullptr;
     int32_t *d_grounds = nullptr;

@@ -1082,6 +1082,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
     if (g->n == 0) rounds = maximal ? 1 : R_max;
     if (rounds < 0) rounds = R_max;
     mm_graph_enqueue_unpermute(g);
+    g->internal_stale = false;
     ML_TRY(hipEventRecord(g->ev_t2, st), "hipEventRecord");
     ML_TRY(hipMemcpyAsync(g->h_lctr, d.ctr, sizeof(int32_t) * LSLOTS * LCTR, hipMemcpyDeviceToHost, st),
            "Unable to read round counters");

@@ -33,6 +33,7 @@ MM_MAX_ROUNDS = 255
 MM_DEFAULT_BARRIER = 0x88B81733
 MM_CREATE_KEEP_ORDER = 1
 MM_CREATE_PACKED_ROWS = 2
+MM_CREATE_NO_HEADS = 4
 MM_LANES_TILES = -1
 MM_KERNEL_CLASSES = 5
 KERNEL_CLASS_NAMES = ("init_select", "request", "respond", "match_select", "counters")
@@ -406,14 +407,16 @@ def _result(st: _Stats, match) -> MatchResult:
 class Graph:
     """Device-resident CSR graph (mm_graph)."""
 
-    def __init__(self, csr: CSR, device: int = 0, keep_order: bool = False, packed_rows: bool = False):
+    def __init__(self, csr: CSR, device: int = 0, keep_order: bool = False, packed_rows: bool = False,
+                 heads: bool = True):
         self.csr = csr
         self.n, self.nnz = csr.n, csr.nnz
         h = ctypes.c_void_p()
         ranges = np.ascontiguousarray(csr.ranges, dtype=np.int32)
         nb = np.ascontiguousarray(csr.neighbours, dtype=np.int32)
         w = None if csr.weights is None else np.ascontiguousarray(csr.weights, dtype=np.float32)
-        flags = (MM_CREATE_KEEP_ORDER if keep_order else 0) | (MM_CREATE_PACKED_ROWS if packed_rows else 0)
+        flags = ((MM_CREATE_KEEP_ORDER if keep_order else 0) | (MM_CREATE_PACKED_ROWS if packed_rows else 0)
+                 | (0 if heads else MM_CREATE_NO_HEADS))
         _check(lib().mm_graph_create_ex(device, self.n, self.nnz, _ptr(ranges), _ptr(nb), _ptr(w), flags,
                                         ctypes.byref(h)))
         self._h = h
