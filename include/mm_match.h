@@ -102,6 +102,11 @@ typedef struct mm_graph_info {
     int32_t max_degree;
     int32_t push_ok;          /* symmetric, strictly ascending rows, symmetric weights */
     int64_t device_bytes;     /* device memory held by the handle */
+    int32_t key_bits;         /* 32: PUSH proposals of the lane scans are 32-bit keys (position of the
+                                 edge in the red vertex's sorted row, then local id: needs
+                                 max_degree < 2^(32 - bits of the largest local id));
+                                 64: (weight, local id) keys; 0: no PUSH */
+    int32_t reserved;
 } mm_graph_info;
 
 /* Last error message of the calling thread ("" when none). */

@@ -69,7 +69,11 @@ struct Dev {
     const int32_t *vbase; // batches: first caller id of each vertex's graph (null: single graph)
     int32_t *req;
     uint8_t *st;
-    unsigned long long *resp;  // [2][n], PUSH only
+    unsigned long long *resp;  // [2][n], PUSH only; with key32 the same memory holds uint32 [2][n]
+    int32_t key32;             // PUSH keys are 32-bit: (position of the proposer in the red vertex's
+                               // sorted row) << locbits | proposer's local id; the scanned rows then
+                               // carry those positions where the weights would be
+    int32_t locbits;
     int4 *lists;               // [2][NLISTS][cap] entries {v, row begin, row end, loc[v]};
                                // classes 0 and 2 split into nb buckets of bcap
     int32_t *ctr;              // [ROUND_SLOTS][CTR_STRIDE] totals; [1],[3] = hub list sizes (atomics)
@@ -261,6 +265,29 @@ __device__ __forceinline__ unsigned long long proposal_key(float w, int32_t v)
     return ((unsigned long long)hi << 32) | (unsigned long long)(0xFFFFFFFFu - (uint32_t)v);
 }
 
+// Proposal keys of round parity p (64-bit, or 32-bit in the same memory).
+__device__ __forceinline__ unsigned long long *resp_slot(const Dev &d, int p)
+{
+    if (d.key32) return reinterpret_cast<unsigned long long *>(reinterpret_cast<uint32_t *>(d.resp) + (size_t)p * d.n);
+    return d.resp + (size_t)p * d.n;
+}
+// Empty key: 0 for the 64-bit max, all ones for the 32-bit min.
+__device__ __forceinline__ void clear_key(const Dev &d, unsigned long long *slot, int32_t v)
+{
+    if (d.key32) reinterpret_cast<uint32_t *>(slot)[v] = 0xFFFFFFFFu;
+    else slot[v] = 0ull;
+}
+// Key at vertex v, widened: 32-bit keys come back with the empty key as 0 and
+// the proposer's local id, complemented, in the low half -- the form of the
+// 64-bit keys, which partner_of decodes.
+__device__ __forceinline__ unsigned long long key_at(const Dev &d, const unsigned long long *slot, int32_t v)
+{
+    if (!d.key32) return slot[v];
+    const uint32_t k = reinterpret_cast<const uint32_t *>(slot)[v];
+    if (k == 0xFFFFFFFFu) return 0ull;
+    return (1ull << 32) | (unsigned long long)(0xFFFFFFFFu - (k & ((1u << d.locbits) - 1u)));
+}
+
 // Writes v's request (or, RESPOND, its accepted proposer) for winner entry
 // (u, w) or none (u < 0).
 template <bool W, bool PUSH, bool RESPOND>
@@ -269,7 +296,13 @@ __device__ __forceinline__ void first_write(const Dev &d, int32_t v, int32_t vlo
 {
     if (u >= 0) {
         d.req[v] = u;
-        if (!RESPOND && PUSH) atomicMax(resp + u, proposal_key<W>(w, vloc));
+        if (!RESPOND && PUSH) {
+            if (d.key32)  // (w holds the position bits of the winning entry)
+                atomicMin(reinterpret_cast<uint32_t *>(resp) + u,
+                          ((uint32_t)__float_as_int(w) << d.locbits) | (uint32_t)vloc);
+            else
+                atomicMax(resp + u, proposal_key<W>(w, vloc));
+        }
     } else {
         d.req[v] = (RESPOND || alive) ? REQ_NONE : REQ_DEAD;
     }
@@ -329,6 +362,29 @@ __device__ __forceinline__ void scan_rest(const Dev &d, int32_t v, int2 rg, int3
 // per thread of the block) and scanned together after the first steps of all
 // rows; a row that finds no room is finished in place.
 extern __shared__ __align__(16) int4 s_deferred[];
+
+// Second pass over the nd rows deferred to s_def: {v (top bit: alive so
+// far), next entry, row end, loc}, G lanes per row.
+template <bool W, bool A8, bool PUSH, bool RESPOND, int G>
+__device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned long long *resp)
+{
+    constexpr int GPW = 64 / G;
+    const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
+    const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (gw * G));
+    const int step = (int)(blockDim.x >> 6) * GPW;
+    for (int i0 = (threadIdx.x >> 6) * GPW; i0 < nd; i0 += step) {
+        const bool valid = i0 + gw < nd;
+        const int4 e = valid ? s_def[i0 + gw] : make_int4(0, 0, 0, 0);
+        const int32_t v = e.x & INT_MAX;
+        bool alive = e.x < 0;
+        const int2 rg = make_int2(e.y, e.z);
+        int32_t win_u = -1;
+        float win_w = 1.0f;
+        scan_rest<W, A8, RESPOND, G>(d, v, rg, rg.x, !valid, alive, win_u, win_w);
+        alive = (__ballot(alive) & gmask) != 0ull;
+        if (valid && gl == 0) first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp);
+    }
+}
 
 template <bool W, bool A8, bool PUSH, bool RESPOND, int G, bool HEAD = false>
 __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp,
@@ -405,19 +461,77 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
         praw = nraw;
     }
     __syncthreads();
-    const int nd = min(s_ndef, DEF_CAP);
-    for (int i0 = (threadIdx.x >> 6) * GPW; i0 < nd; i0 += step) {
-        const bool valid = i0 + gw < nd;
-        const int4 e = valid ? s_def[i0 + gw] : none;
-        const int32_t v = e.x & INT_MAX;
-        bool alive = e.x < 0;
-        const int2 rg = make_int2(e.y, e.z);
+    deferred_pass<W, A8, PUSH, RESPOND, G>(d, s_def, min(s_ndef, DEF_CAP), resp);
+}
+
+// Round-1 scans over the head copy, one lane per row: a lane reads its row's
+// 4 leading entries (two 16-byte loads) and gathers their 4 states at once,
+// so a wavefront has 64 rows in flight with two dependent memory steps each.
+// Rows with no candidate among them and more entries go to the second pass
+// (G2 lanes per row).  Weighted rows only (the head copy exists for those).
+template <bool PUSH, int G2>
+__device__ void head_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp,
+                                 unsigned long long &nnz)
+{
+    int4 *s_def = s_deferred;
+    const int DEF_CAP = (int)blockDim.x;
+    __shared__ int32_t s_ndef;
+    if (count <= 0) return;
+    if (threadIdx.x == 0) s_ndef = 0;
+    __syncthreads();
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    for (int base = (int)(threadIdx.x & ~63u); base < count; base += (int)blockDim.x) {
+        const int i = base + (int)(threadIdx.x & 63);
+        const bool valid = i < count;
+        const int4 e = load_ent(list + min(i, count - 1));
+        const v4i *hp = reinterpret_cast<const v4i *>(d.head + (size_t)e.x * 4);
+        const v4i h01 = __builtin_nontemporal_load(hp), h23 = __builtin_nontemporal_load(hp + 1);
+        const int32_t deg = valid ? e.z - e.y : 0;
+        const int2 raw[4] = {make_int2(h01.x, h01.y), make_int2(h01.z, h01.w), make_int2(h23.x, h23.y),
+                             make_int2(h23.z, h23.w)};
+        int32_t u[4];
+        bool ok[4];
+        float w[4];
+        uint8_t sv[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) decode_raw<true>(raw[k], u[k], ok[k], w[k]);
+        // (entries past the row end gather vertex 0's slot)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sv[k] = d.st[k < deg ? u[k] : 0];
+        bool alive = false;
         int32_t win_u = -1;
         float win_w = 1.0f;
-        scan_rest<W, A8, RESPOND, G>(d, v, rg, rg.x, !valid, alive, win_u, win_w);
-        alive = (__ballot(alive) & gmask) != 0ull;
-        if (valid && gl == 0) first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < deg && win_u < 0 && candidate_of<true, false>(0, sv[k], 0, ok[k], alive)) {
+                win_u = u[k];
+                win_w = w[k];
+            }
+        bool deferred = false;
+        if (valid && win_u < 0 && deg > 4) {
+            const int slot = atomicAdd(&s_ndef, 1);
+            if (slot < DEF_CAP) {
+                s_def[slot] = make_int4(e.x | (alive ? INT_MIN : 0), e.y + 4, e.z, e.w);
+                deferred = true;
+            } else {
+                // no room left: finish the row here
+                for (int32_t j = e.y + 4; j < e.z && win_u < 0; ++j) {
+                    int32_t uj;
+                    bool okj;
+                    float wj;
+                    load_id<true>(d, j, uj, okj, wj);
+                    if (candidate_of<true, false>(0, d.st[uj], 0, okj, alive)) {
+                        win_u = uj;
+                        win_w = wj;
+                    }
+                }
+            }
+        }
+        if (valid && !deferred) first_write<true, PUSH, false>(d, e.x, e.w, win_u, win_w, alive, resp);
+        if (valid) nnz += (unsigned long long)deg;
     }
+    __syncthreads();
+    deferred_pass<true, true, PUSH, false, G2>(d, s_def, min(s_ndef, DEF_CAP), resp);
 }
 
 // Rows longer than the hub degree: one block per row, blockDim entries per
@@ -558,7 +672,7 @@ __global__ void k_init_select(Dev d)
                 cls = 2 * c + hub;
                 if (hub) hub_deg_sum += (unsigned long long)deg;
                 else deg_sum += (unsigned long long)deg;
-                if (PUSH && c == ST_RED) d.resp[(size_t)parity * d.n + v] = 0ull;
+                if (PUSH && c == ST_RED) clear_key(d, resp_slot(d, parity), v);
             }
             bucket_append(cls == L_BLUE_LO, ent, blue, s_cnt + 0);
             bucket_append(cls == L_RED_LO, ent, red, s_cnt + 1);
@@ -588,7 +702,7 @@ template <bool W, bool A8, bool PUSH, int G, bool HEAD = false>
 __global__ void k_request(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
-    unsigned long long *resp = PUSH ? d.resp + (size_t)parity * d.n : nullptr;
+    unsigned long long *resp = PUSH ? resp_slot(d, parity) : nullptr;
     unsigned long long nnz = 0;
     if ((int)blockIdx.x >= d.nb) {
         first_scan_hubs<W, A8, PUSH, false>(d, list_ptr(d, parity, L_BLUE_HI), d.ctr[r * CTR_STRIDE + L_BLUE_HI],
@@ -597,8 +711,12 @@ __global__ void k_request(Dev d, int r, int hub_blocks)
         return;
     }
     const int b = bucket_of_block(d);
-    first_scan_bucket<W, A8, PUSH, false, G, HEAD>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap,
-                                                   ctrb_ptr(d, r, 0)[b], resp, nnz);
+    if (HEAD)
+        head_scan_bucket<PUSH, 8>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap, ctrb_ptr(d, r, 0)[b],
+                                  resp, nnz);
+    else
+        first_scan_bucket<W, A8, PUSH, false, G>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap,
+                                                 ctrb_ptr(d, r, 0)[b], resp, nnz);
     if (d.count_nnz) {
         nnz = block_sum(nnz);
         if (threadIdx.x == 0) ctrb64_ptr(d, r, 1)[b] = nnz;
@@ -853,7 +971,7 @@ __device__ __forceinline__ int settle_vertex(const Dev &d, int32_t v, int32_t vl
     }
     const int c = colour_of((uint32_t)vloc, seed, d.barrier);
     if (c != colour) d.st[v] = (uint8_t)c;
-    if (PUSH && c == ST_RED) nresp[v] = 0ull;
+    if (PUSH && c == ST_RED) clear_key(d, nresp, v);
     return 2 * c + hub;
 }
 
@@ -887,7 +1005,7 @@ __device__ __forceinline__ void partner_words(const Dev &d, int32_t v, int colou
 {
     a = -1;
     k = 0ull;
-    if (PUSH && colour == ST_RED) k = resp[v];
+    if (PUSH && colour == ST_RED) k = key_at(d, resp, v);
     else a = d.req[v];
 }
 template <bool PUSH>
@@ -897,7 +1015,7 @@ __device__ __forceinline__ void partner_words2(const Dev &d, int colour, const u
     pl = -1;
     if (a < 0) return;
     if (PUSH) {
-        if (colour == ST_BLUE) k = resp[a];
+        if (colour == ST_BLUE) k = key_at(d, resp, a);
     } else {
         k = (unsigned long long)(uint32_t)d.req[a];
         pl = d.loc[a];
@@ -921,8 +1039,8 @@ __global__ void k_match_select(Dev d, int r, int do_select)
     const int4 *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
     int4 *nblue = list_ptr(d, nparity, L_BLUE_LO) + (size_t)b * d.bcap;
     int4 *nred = list_ptr(d, nparity, L_RED_LO) + (size_t)b * d.bcap;
-    const unsigned long long *resp = PUSH ? d.resp + (size_t)parity * d.n : nullptr;
-    unsigned long long *nresp = PUSH ? d.resp + (size_t)nparity * d.n : nullptr;
+    const unsigned long long *resp = PUSH ? resp_slot(d, parity) : nullptr;
+    unsigned long long *nresp = PUSH ? resp_slot(d, nparity) : nullptr;
     const uint32_t seed = do_select ? d.seeds[r] : 0u;
     int32_t n_matched = 0, n_dead = 0;
     unsigned long long deg_sum = 0, hub_deg_sum = 0;
@@ -1135,7 +1253,7 @@ void release(mm_graph *g)
     if (!g) return;
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
-    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_head,  g->d_match, g->d_req,   g->d_st,
+    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_adjp,   g->d_head,  g->d_match, g->d_req,   g->d_st,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
                     g->d_unperm, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_lcnt, g->d_llist, g->d_params, g->d_vbase, g->d_chk, g->d_cpart,
@@ -1398,6 +1516,17 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     std::vector<int32_t> h_nbrs((size_t)nnz_dev, 0);
     std::vector<int2> h_adj;
     if (g->weighted) h_adj.assign((size_t)nnz_dev, make_int2(0, 0));
+    // 32-bit proposal keys need the position of every edge in the other end's
+    // sorted row (PUSH graphs: symmetric, ascending rows, symmetric weights)
+    // and room for (position << L) | local id with L bits of local id.
+    int32_t max_loc = 0;
+    for (int32_t i = 0; i < n; ++i) max_loc = std::max(max_loc, h_loc[(size_t)i]);
+    int locbits = 1;
+    while (locbits < 31 && ((int64_t)1 << locbits) <= (int64_t)max_loc) ++locbits;
+    const bool key32 = g->weighted && g->push_ok_weights && n > 0 && locbits < 31 &&
+                       (int64_t)g->max_degree < ((int64_t)1 << (32 - locbits));
+    std::vector<int32_t> h_spos;  // sorted position of each row entry in CSR order
+    if (key32) h_spos.assign((size_t)nnz_dev, 0);
     mm::parallel_for_dynamic(n, 1024, [&](int64_t b, int64_t e, unsigned) {
         std::vector<int32_t> perm;
         for (int64_t i = b; i < e; ++i) {
@@ -1424,9 +1553,34 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
                 int32_t wb;
                 std::memcpy(&wb, &wk, sizeof(wb));
                 h_adj[(size_t)dst + k] = make_int2(wk > -1.0f ? id : ~id, wb);
+                if (key32) h_spos[(size_t)dst + perm[(size_t)k]] = k;
             }
         }
     });
+    // Rows of the plain scans with 32-bit keys: {id, position of this vertex
+    // in the neighbour's sorted row}.  Row u's entry at sorted position p names
+    // v; it gives the position field of v's entry for u, found by a binary
+    // search of the ascending CSR row of v.
+    std::vector<int2> h_adjp;
+    if (key32) {
+        h_adjp = h_adj;
+        mm::parallel_for_dynamic(n, 1024, [&](int64_t b, int64_t e, unsigned) {
+            for (int64_t u = b; u < e; ++u) {
+                const int32_t ou = old_of(u);
+                const int32_t dst = h_ranges[2 * (size_t)u], deg = h_ranges[2 * (size_t)u + 1] - dst;
+                for (int32_t p = 0; p < deg; ++p) {
+                    const int32_t x = h_adj[(size_t)dst + p].x, v = x >= 0 ? x : ~x;
+                    const int32_t ov = old_of(v);
+                    const int32_t *rb = nbrs + ranges[2 * (size_t)ov], *re = nbrs + ranges[2 * (size_t)ov + 1];
+                    const int32_t c = (int32_t)(std::lower_bound(rb, re, ou) - rb);
+                    const size_t at = (size_t)h_ranges[2 * (size_t)v] + (size_t)h_spos[(size_t)h_ranges[2 * (size_t)v] + c];
+                    h_adjp[at].y = p;
+                }
+            }
+        });
+        g->locbits = locbits;
+    }
+    const std::vector<int2> &h_scan = key32 ? h_adjp : h_adj;  // rows the plain scans read
     if (n > 0 && (hipMemcpy(g->d_ranges, h_ranges.data(), sizeof(int2) * nn, hipMemcpyHostToDevice) != hipSuccess ||
                   hipMemcpy(g->d_loc, h_loc.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess))
         return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
@@ -1437,7 +1591,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         mm::parallel_for_dynamic(n, 4096, [&](int64_t b, int64_t e, unsigned) {
             for (int64_t i = b; i < e; ++i) {
                 const int32_t lo = h_ranges[2 * (size_t)i], hi = h_ranges[2 * (size_t)i + 1];
-                for (int32_t k = 0; k < 4 && lo + k < hi; ++k) h_head[4 * (size_t)i + k] = h_adj[(size_t)lo + k];
+                for (int32_t k = 0; k < 4 && lo + k < hi; ++k) h_head[4 * (size_t)i + k] = h_scan[(size_t)lo + k];
             }
         });
         if ((rc = dev_alloc(g, &g->d_head, 4 * nn, "row heads")) != MM_OK) return fail(rc);
@@ -1450,6 +1604,13 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
             (g->weighted &&
              hipMemcpy(g->d_adj, h_adj.data(), sizeof(int2) * (size_t)nnz_dev, hipMemcpyHostToDevice) != hipSuccess))
             return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+        if (key32) {
+            if ((rc = dev_alloc(g, &g->d_adjp, (size_t)nnz_dev, "sorted rows with key positions")) != MM_OK)
+                return fail(rc);
+            if (hipMemcpy(g->d_adjp, h_adjp.data(), sizeof(int2) * (size_t)nnz_dev, hipMemcpyHostToDevice) !=
+                hipSuccess)
+                return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+        }
     }
     if (vbase_old && n > 0) {
         if ((rc = dev_alloc(g, &g->d_vbase, nn, "graph bases")) != MM_OK) return fail(rc);
@@ -1669,6 +1830,10 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.nbrs = g->d_nbrs;
     d.adj = g->d_adj;
     d.head = c.HEAD ? g->d_head : nullptr;
+    // 32-bit keys with the first-candidate scans, when the graph has the rows for them
+    d.key32 = c.PUSH && c.A8 && G != 0 && g->d_adjp != nullptr;
+    d.locbits = g->locbits;
+    if (d.key32) d.adj = g->d_adjp;
     // Vertices write their final values straight into the caller-order result
     // as they settle, so no permutation pass follows the last round.
     d.out = g->relabelled ? g->d_match_out : g->d_match;
@@ -2071,6 +2236,8 @@ int32_t mm_graph_get_info(const mm_graph *g, mm_graph_info *info)
     info->max_degree = g->max_degree;
     info->push_ok = g->weighted ? g->push_ok_weights : g->push_ok_struct;
     info->device_bytes = g->device_bytes;
+    info->key_bits = !info->push_ok ? 0 : (g->d_adjp ? 32 : 64);
+    info->reserved = 0;
     return MM_OK;
 }
 

@@ -35,7 +35,11 @@ struct mm_graph {
     int2 *d_ranges = nullptr;
     int32_t *d_nbrs = nullptr;
     int2 *d_adj = nullptr;
-    int2 *d_head = nullptr;   // weighted: entries 0..3 of each sorted adj row, 4 per vertex (round-1 scans)
+    int2 *d_adjp = nullptr;   // adj with each weight replaced by the edge's position in the other end's
+                              // sorted row (32-bit proposal keys; PUSH graphs that fit them)
+    int locbits = 0;          // bits of local id in a 32-bit proposal key
+    int2 *d_head = nullptr;   // weighted: entries 0..3 of each sorted row (adjp when present, else adj),
+                              // 4 per vertex (round-1 scans)
     int32_t *d_match = nullptr;
     int32_t *d_req = nullptr;
     uint8_t *d_st = nullptr;

@@ -95,6 +95,8 @@ class _Info(ctypes.Structure):
         ("max_degree", ctypes.c_int32),
         ("push_ok", ctypes.c_int32),
         ("device_bytes", ctypes.c_int64),
+        ("key_bits", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

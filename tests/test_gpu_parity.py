@@ -162,6 +162,34 @@ def test_rmat_hub_path_bitexact(hub, keep):
                 check_against_oracle(csr, g, weighted, True, hub_degree=hub, respond=respond, lanes=lanes)
 
 
+def test_proposal_key_widths():
+    """PUSH proposals use 32-bit keys (position in the red vertex's sorted
+    row, then local id) when the degrees leave room for them, 64-bit
+    (weight, local id) keys otherwise; both match the oracle on lane scans,
+    the round-1 head scan and the hub path.  A hub of degree 5000 in a graph of
+    2^20 vertices (20-bit ids leave 12 bits of position) forces the wide keys."""
+    rng = np.random.default_rng(21)
+    n = 1 << 20
+    ring = np.stack([np.arange(n), (np.arange(n) + 1) % n], axis=1)
+    chords = rng.integers(0, n, size=(n // 2, 2))
+    star = np.stack([np.zeros(5000, np.int64), 1 + rng.choice(n - 1, 5000, replace=False)], axis=1)
+    e = np.concatenate([ring, chords, star])
+    e = e[e[:, 0] != e[:, 1]]
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    # ties: a few weights repeat, so the position part of a key must break them as the oracle does
+    w = (rng.integers(0, 1 << 10, size=len(e)) + 1).astype(np.float32) / np.float32(1 << 10)
+    wide = CSR.from_edges(n, e, w)
+    narrow = mmamd.gen_rgg(1 << 16, seed=22, weighted=True)
+    for csr, bits in ((wide, 64), (narrow, 32)):
+        say("key width", bits)
+        g = Graph(csr)
+        info = g.info()
+        assert info["push_ok"] and info["key_bits"] == bits
+        for lanes, hub in ((8, 256), (4, 4096), (-1, 256)):
+            check_against_oracle(csr, g, True, True, lanes=lanes, hub_degree=hub)
+        g.close()
+
+
 def test_batch_equals_single_graph_path():
     say("batch of mixed graphs")
     rng = np.random.default_rng(3)
