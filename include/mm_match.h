@@ -135,8 +135,9 @@ int32_t mm_graph_create(int32_t device, int32_t n, int64_t nnz, const int32_t *r
 /* MM_CREATE_NO_HEADS skips the head copy of a weighted graph.  By default the
  * handle also keeps the first 4 entries of every weight-sorted row in a dense
  * array (32 bytes per vertex): the first round's scans, which visit about half
- * of all rows and usually stop within 4 entries, read that array instead of the
- * rows themselves, 4 lanes per row.  Results are the same either way. */
+ * of all rows and usually stop within 4 entries, read that array (one lane per
+ * row, all coalesced) instead of the rows themselves.  Results are the same
+ * either way. */
 #define MM_CREATE_NO_HEADS 4u
 int32_t mm_graph_create_ex(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges,
                            const int32_t *neighbours, const float *weights, uint32_t flags, mm_graph **out);

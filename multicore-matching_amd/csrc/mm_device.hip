@@ -74,6 +74,7 @@ struct Dev {
                                // sorted row) << locbits | proposer's local id; the scanned rows then
                                // carry those positions where the weights would be
     int32_t locbits;
+    int32_t listless1;         // round 1 runs over the buckets' id ranges: no round-1 lists of short rows
     int4 *lists;               // [2][NLISTS][cap] entries {v, row begin, row end, loc[v]};
                                // classes 0 and 2 split into nb buckets of bcap
     int32_t *ctr;              // [ROUND_SLOTS][CTR_STRIDE] totals; [1],[3] = hub list sizes (atomics)
@@ -182,20 +183,14 @@ __device__ __forceinline__ void decode_raw(int2 e, int32_t &u, bool &valid, floa
     }
 }
 
-// First row step of list entry e for lane gl: entry e.y + gl of the row, from
-// the dense head copy when HEAD (entries 0..3 of v = e.x's row, 4 per vertex).
+// First row step of list entry e for lane gl: entry e.y + gl of the row.
 // Always a valid address -- clamped into the row, or entry 0 for an empty row
 // (the caller masks lanes past the row end) -- so the load can be issued
 // unconditionally: a load under a branch makes the compiler wait for every
 // load in flight at the first use after it.
-template <bool A8, bool HEAD>
+template <bool A8>
 __device__ __forceinline__ int2 first_raw(const Dev &d, const int4 &e, int gl)
 {
-    typedef int v2i __attribute__((ext_vector_type(2)));
-    if (HEAD) {
-        const v2i h = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(d.head + (size_t)e.x * 4 + gl));
-        return make_int2(h.x, h.y);
-    }
     const int32_t j = e.y + gl < e.z ? e.y + gl : (e.z > e.y ? e.z - 1 : 0);
     return load_raw<A8>(d, j);
 }
@@ -308,18 +303,6 @@ __device__ __forceinline__ void first_write(const Dev &d, int32_t v, int32_t vlo
     }
 }
 
-// G lanes per row, G entries per step; a group stops at its first candidate
-// (lowest lane of the first step that has one).  Wave-uniform control flow.
-// Software-pipelined: while a group scans its row, the list entry two steps
-// ahead and the first row chunk of the next entry are already in flight, so
-// a row that settles in its first chunk (most of them) costs one dependent
-// memory trip (the neighbour-state gathers) instead of two.  Vector loads
-// return in issue order, so the gathers of the first step are issued before
-// the prefetches: waiting for them then leaves the prefetches (HBM) in flight
-// instead of waiting for those too.  HEAD (G == 4):
-// the first step reads the row's 4 leading entries from the dense head copy
-// (32 bytes per vertex, so the rows of one wave share lines) instead of the
-// row's own line.
 // Steps 1, 2, ... of a row scan (entries from base on), G lanes per row; the
 // group stops at its first candidate.  Wave-uniform control flow: every lane
 // of the wavefront must call it (done lanes idle).
@@ -386,11 +369,19 @@ __device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned 
     }
 }
 
-template <bool W, bool A8, bool PUSH, bool RESPOND, int G, bool HEAD = false>
+// G lanes per row, G entries per step; a group stops at its first candidate
+// (lowest lane of the first step that has one).  Wave-uniform control flow.
+// Software-pipelined: while a group scans its row, the list entry two steps
+// ahead and the first row chunk of the next entry are already in flight, so
+// a row that settles in its first chunk (most of them) costs one dependent
+// memory trip (the neighbour-state gathers) instead of two.  Vector loads
+// return in issue order, so the gathers of the first step are issued before
+// the prefetches: waiting for them then leaves the prefetches (HBM) in flight
+// instead of waiting for those too.
+template <bool W, bool A8, bool PUSH, bool RESPOND, int G>
 __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp,
                                   unsigned long long &nnz)
 {
-    static_assert(!HEAD || (G == 4 && A8), "head steps are 4 sorted entries");
     constexpr int GPW = 64 / G;
     int4 *s_def = s_deferred;
     const int DEF_CAP = (int)blockDim.x;
@@ -408,7 +399,7 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
     int first = (threadIdx.x >> 6) * GPW;
     int4 cur = ent_at(first + gw);
     int4 nxt = ent_at(first + step + gw);
-    int2 praw = first_raw<A8, HEAD>(d, cur, gl);  // first step of cur's row
+    int2 praw = first_raw<A8>(d, cur, gl);  // first step of cur's row
     for (; first < count; first += step) {
         const bool valid = first + gw < count;
         const int4 e = cur;
@@ -427,7 +418,7 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
         // keep the scheduler from hoisting the prefetches above the gathers
         __builtin_amdgcn_sched_barrier(0);
         const int4 nn = ent_at(first + 2 * step + gw);
-        const int2 nraw = first_raw<A8, HEAD>(d, nxt, gl);
+        const int2 nraw = first_raw<A8>(d, nxt, gl);
         // step 0
         int32_t win_u = -1;
         float win_w = 1.0f;
@@ -464,29 +455,36 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
     deferred_pass<W, A8, PUSH, RESPOND, G>(d, s_def, min(s_ndef, DEF_CAP), resp);
 }
 
-// Round-1 scans over the head copy, one lane per row: a lane reads its row's
-// 4 leading entries (two 16-byte loads) and gathers their 4 states at once,
-// so a wavefront has 64 rows in flight with two dependent memory steps each.
-// Rows with no candidate among them and more entries go to the second pass
-// (G2 lanes per row).  Weighted rows only (the head copy exists for those).
+// Round-1 scans over the head copy, one lane per row, straight over the
+// bucket's id range [lo, hi) (every vertex is active in round 1): a lane reads
+// its vertex's state, row range, id and the row's 4 leading entries (two
+// 16-byte loads) at once, all coalesced, then gathers the 4 neighbour states
+// at once -- two dependent memory steps for 64 rows per wavefront.  Blue rows
+// with no candidate among those 4 and more entries go to the second pass (G2
+// lanes per row); rows above the hub degree are the hub blocks' work.
+// Weighted rows only (the head copy exists for those).
 template <bool PUSH, int G2>
-__device__ void head_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp,
-                                 unsigned long long &nnz)
+__device__ void head_scan_range(const Dev &d, int64_t lo, int64_t hi, unsigned long long *resp,
+                                unsigned long long &nnz)
 {
     int4 *s_def = s_deferred;
     const int DEF_CAP = (int)blockDim.x;
     __shared__ int32_t s_ndef;
-    if (count <= 0) return;
+    if (hi <= lo) return;
     if (threadIdx.x == 0) s_ndef = 0;
     __syncthreads();
     typedef int v4i __attribute__((ext_vector_type(4)));
-    for (int base = (int)(threadIdx.x & ~63u); base < count; base += (int)blockDim.x) {
-        const int i = base + (int)(threadIdx.x & 63);
-        const bool valid = i < count;
-        const int4 e = load_ent(list + min(i, count - 1));
-        const v4i *hp = reinterpret_cast<const v4i *>(d.head + (size_t)e.x * 4);
+    for (int64_t base = lo + (threadIdx.x & ~63u); base < hi; base += blockDim.x) {
+        const int64_t vi = base + (threadIdx.x & 63);
+        const int32_t v = (int32_t)min(vi, hi - 1);
+        const uint8_t sv0 = d.st[v];
+        const int2 rg = d.ranges[v];
+        const int32_t vloc = d.loc[v];
+        const v4i *hp = reinterpret_cast<const v4i *>(d.head + (size_t)v * 4);
         const v4i h01 = __builtin_nontemporal_load(hp), h23 = __builtin_nontemporal_load(hp + 1);
-        const int32_t deg = valid ? e.z - e.y : 0;
+        const int32_t deg0 = rg.y - rg.x;
+        const bool valid = vi < hi && sv0 == ST_BLUE && deg0 <= d.hub_deg;
+        const int32_t deg = valid ? deg0 : 0;
         const int2 raw[4] = {make_int2(h01.x, h01.y), make_int2(h01.z, h01.w), make_int2(h23.x, h23.y),
                              make_int2(h23.z, h23.w)};
         int32_t u[4];
@@ -511,11 +509,11 @@ __device__ void head_scan_bucket(const Dev &d, const int4 *list, int32_t count, 
         if (valid && win_u < 0 && deg > 4) {
             const int slot = atomicAdd(&s_ndef, 1);
             if (slot < DEF_CAP) {
-                s_def[slot] = make_int4(e.x | (alive ? INT_MIN : 0), e.y + 4, e.z, e.w);
+                s_def[slot] = make_int4(v | (alive ? INT_MIN : 0), rg.x + 4, rg.y, vloc);
                 deferred = true;
             } else {
                 // no room left: finish the row here
-                for (int32_t j = e.y + 4; j < e.z && win_u < 0; ++j) {
+                for (int32_t j = rg.x + 4; j < rg.y && win_u < 0; ++j) {
                     int32_t uj;
                     bool okj;
                     float wj;
@@ -527,7 +525,7 @@ __device__ void head_scan_bucket(const Dev &d, const int4 *list, int32_t count, 
                 }
             }
         }
-        if (valid && !deferred) first_write<true, PUSH, false>(d, e.x, e.w, win_u, win_w, alive, resp);
+        if (valid && !deferred) first_write<true, PUSH, false>(d, v, vloc, win_u, win_w, alive, resp);
         if (valid) nnz += (unsigned long long)deg;
     }
     __syncthreads();
@@ -597,6 +595,13 @@ __device__ __forceinline__ void bucket_append(bool pred, int4 value, int4 *bucke
     if (lane == leader) base = atomicAdd(s_count, (int)__popcll(mask));
     base = __shfl(base, leader);
     if (pred) bucket[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
+// bucket_append without the entry: counts the lanes with pred set.
+__device__ __forceinline__ void bucket_count(bool pred, int32_t *s_count)
+{
+    const unsigned long long mask = __ballot(pred);
+    if (mask != 0ull && __lane_id() == __ffsll((long long)mask) - 1) atomicAdd(s_count, (int)__popcll(mask));
 }
 
 template <typename T>
@@ -674,8 +679,13 @@ __global__ void k_init_select(Dev d)
                 else deg_sum += (unsigned long long)deg;
                 if (PUSH && c == ST_RED) clear_key(d, resp_slot(d, parity), v);
             }
-            bucket_append(cls == L_BLUE_LO, ent, blue, s_cnt + 0);
-            bucket_append(cls == L_RED_LO, ent, red, s_cnt + 1);
+            if (d.listless1) {
+                bucket_count(cls == L_BLUE_LO, s_cnt + 0);
+                bucket_count(cls == L_RED_LO, s_cnt + 1);
+            } else {
+                bucket_append(cls == L_BLUE_LO, ent, blue, s_cnt + 0);
+                bucket_append(cls == L_RED_LO, ent, red, s_cnt + 1);
+            }
             wave_append(cls == L_BLUE_HI, ent, list_ptr(d, parity, L_BLUE_HI), d.ctr + 1 * CTR_STRIDE + L_BLUE_HI);
             wave_append(cls == L_RED_HI, ent, list_ptr(d, parity, L_RED_HI), d.ctr + 1 * CTR_STRIDE + L_RED_HI);
         }
@@ -712,8 +722,7 @@ __global__ void k_request(Dev d, int r, int hub_blocks)
     }
     const int b = bucket_of_block(d);
     if (HEAD)
-        head_scan_bucket<PUSH, 8>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap, ctrb_ptr(d, r, 0)[b],
-                                  resp, nnz);
+        head_scan_range<PUSH, 8>(d, (int64_t)b * d.bcap, min((int64_t)d.n, (int64_t)(b + 1) * d.bcap), resp, nnz);
     else
         first_scan_bucket<W, A8, PUSH, false, G>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap,
                                                  ctrb_ptr(d, r, 0)[b], resp, nnz);
@@ -1026,15 +1035,18 @@ __device__ __forceinline__ void partner_words2(const Dev &d, int colour, const u
 // resolves bucket b of round r's lists (and a share of the hub lists),
 // recolours the vertices still active and files them into bucket b of round
 // r+1's lists.
-template <bool PUSH>
+template <bool PUSH, bool LL = false>
 __global__ void k_match_select(Dev d, int r, int do_select)
 {
     __shared__ int32_t s_cnt[2];
     const int parity = r & 1, nparity = parity ^ 1, b = bucket_of_block(d);
     if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const int32_t c0 = ctrb_ptr(d, r, 0)[b], c1 = ctrb_ptr(d, r, 1)[b];
-    const int32_t total = c0 + c1;
+    // LL (round 1 without lists): the items are the bucket's ids, every one
+    // active; rows above the hub degree are left to the hub part below.
+    const int64_t id0 = (int64_t)b * d.bcap;
+    const int32_t c0 = LL ? 0 : ctrb_ptr(d, r, 0)[b], c1 = LL ? 0 : ctrb_ptr(d, r, 1)[b];
+    const int32_t total = LL ? (int32_t)max((int64_t)0, min((int64_t)d.n, id0 + d.bcap) - id0) : c0 + c1;
     const int4 *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
     const int4 *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
     int4 *nblue = list_ptr(d, nparity, L_BLUE_LO) + (size_t)b * d.bcap;
@@ -1056,12 +1068,24 @@ __global__ void k_match_select(Dev d, int r, int do_select)
             const int t = base + q * blockDim.x + (threadIdx.x & 63);
             col[q] = -1;
             e[q] = make_int4(0, 0, 0, 0);
-            if (t < total) {
+            if (LL) {
+                if (t < total) {
+                    const int32_t vv = (int32_t)(id0 + t);
+                    const int2 rg = d.ranges[vv];
+                    e[q] = make_int4(vv, rg.x, rg.y, d.loc[vv]);
+                    col[q] = d.st[vv];
+                }
+            } else if (t < total) {
                 col[q] = t < c0 ? ST_BLUE : ST_RED;
                 e[q] = load_ent(t < c0 ? blue + t : red + (t - c0));
             }
             v[q] = e[q].x;
             vl[q] = e[q].w;
+        }
+        if (LL) {
+#pragma unroll
+            for (int q = 0; q < MU; ++q)
+                if (col[q] >= 0 && e[q].z - e[q].y > d.hub_deg) col[q] = -1;
         }
 #pragma unroll
         for (int q = 0; q < MU; ++q) {
@@ -1664,7 +1688,7 @@ size_t deferred_smem_bytes(int block) { return sizeof(int4) * (size_t)block; }
 
 struct RunCfg {
     bool W, A8, PUSH;
-    bool HEAD;  // round 1 scans the head copy, 4 lanes per row
+    bool HEAD;  // round 1 runs without lists, one lane per row over the head copy
     int G, U, block, hub_deg, nb, bcap, hub_blocks;
 };
 
@@ -1792,8 +1816,9 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     if (o->respond < 0 || o->respond > MM_RESPOND_PUSH) return mm::set_error(MM_ERR_INVALID, "mm_match: bad respond mode");
     c.PUSH = o->respond == MM_RESPOND_PUSH || (o->respond == MM_RESPOND_AUTO && push_ok);
     c.G = G;
-    // The head copy serves the first round of weighted scans (not the tiles).
-    c.HEAD = c.A8 && g->d_head != nullptr && G != 0;
+    // The head copy serves the first round of weighted scans with PUSH
+    // responses (not the tiles); that round then runs without lists.
+    c.HEAD = c.A8 && c.PUSH && g->d_head != nullptr && G != 0;
     c.U = o->entries_per_thread > 0 ? o->entries_per_thread : 4;
     if (c.U != 4 && c.U != 8 && c.U != 16)
         return mm::set_error(MM_ERR_INVALID, "mm_match: entries_per_thread must be 4, 8 or 16");
@@ -1832,6 +1857,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.head = c.HEAD ? g->d_head : nullptr;
     // 32-bit keys with the first-candidate scans, when the graph has the rows for them
     d.key32 = c.PUSH && c.A8 && G != 0 && g->d_adjp != nullptr;
+    d.listless1 = c.HEAD ? 1 : 0;
     d.locbits = g->locbits;
     if (d.key32) d.adj = g->d_adjp;
     // Vertices write their final values straight into the caller-order result
@@ -1922,8 +1948,11 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
             const int do_select = rr < R_run;
             MM_LAUNCH(K_REQUEST, launch_request(c, d, rr, g->stream));
             if (!c.PUSH) MM_LAUNCH(K_RESPOND, launch_respond(c, d, rr, g->stream));
-            if (c.PUSH) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL(k_match_select<true>, dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
-            else MM_LAUNCH(K_MATCH, hipLaunchKernelGGL(k_match_select<false>, dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
+            const bool ll = rr == 1 && d.listless1;
+            if (c.PUSH && ll) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
+            else if (c.PUSH) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
+            else if (ll) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
+            else MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, false>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
         }
         if (maximal) return enq_reduce(r0, std::min(r1 + 1, R_run));
         return MM_OK;
