@@ -51,16 +51,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(res: mmamd.MatchResult, rounds: int, push: bool) -> dict:
+def kernel_bytes(res: mmamd.MatchResult, rounds: int, push: bool, key_bytes: int = 8) -> dict:
     """Algorithmic HBM bytes per kernel class over the first `rounds` rounds.
 
-    request: per blue vertex 4 (list entry) + 8 (range) + 4 (request write)
-             + 8 per row entry (neighbour id + weight); in PUSH mode + 8 for the
-             proposal's atomic on the red vertex.
-    match_select: per active vertex 4 (list entry) + 4 (request read) + 8
-             (partner key / request read) + 4 (match write) + 1 (state write)
-             + 4 (next list entry).
-    init_select: per vertex 8 (range) + 4 (match write) + 1 (state) + 4 (list).
+    request: per blue vertex 4 (id) + 8 (range) + 4 (request write) + 8 per
+             row entry (neighbour id + weight or key position); in PUSH mode
+             + key_bytes (4 or 8) for the proposal's atomic on the red vertex.
+    match_select: per active vertex 4 (id) + 4 (request or own key read) +
+             key_bytes (partner key) + 4 (result write) + 1 (state write) + 4
+             (next list entry).
+    init_select: per vertex 4 (id) + 8 (range) + 1 (state) + key_bytes / 2
+             (empty keys for the red half).
     respond (PULL): per red vertex 4 + 8 + 4 + 8 per row entry.
     Gathers of neighbour state are assumed to hit L2/MALL (not counted)."""
     k = min(rounds, len(res.blue))
@@ -68,9 +69,9 @@ def kernel_bytes(res: mmamd.MatchResult, rounds: int, push: bool) -> dict:
     act, annz = res.active[:k].astype(np.float64), res.active_nnz[:k].astype(np.float64)
     red, rnnz = act - blue, annz - bnnz
     out = {
-        "request": float(np.sum(blue * (16 + (8 if push else 0)) + 8 * bnnz)),
-        "match_select": float(np.sum(act * 25)),
-        "init_select": float(res.active[0] * 17) if k else 0.0,
+        "request": float(np.sum(blue * (16 + (key_bytes if push else 0)) + 8 * bnnz)),
+        "match_select": float(np.sum(act * (17 + key_bytes))),
+        "init_select": float(res.active[0] * (13 + key_bytes / 2)) if k else 0.0,
         "respond": 0.0 if push else float(np.sum(red * 16 + 8 * rnnz)),
     }
     out["round_total"] = float(np.sum(act * (8 + 16) + 8 * annz))
@@ -218,7 +219,7 @@ def run_single(args, csr, name: str, workload_desc: str) -> dict:
     prof = g.match(kind=mmamd.MM_WEIGHTED, maximal=False, max_rounds=r99, seeds=seeds, fetch=False, profile=True,
                    **tiles, **base)
     push = prof.respond == mmamd.MM_RESPOND_PUSH
-    kb = kernel_bytes(prof, r99, push)
+    kb = kernel_bytes(prof, r99, push, key_bytes=max(4, g.info()["key_bits"] // 8))
     kms = {k: v for k, v in prof.kernel_ms.items() if k != "counters"}
     dom = max(kms, key=kms.get)
     launches = prof.kernel_launches[dom]
@@ -304,7 +305,8 @@ def run_batch(args) -> dict:
     log(f"checked {len(pick)} graphs against the single-graph path: {bad} differ; device check {chk} "
         f"({int(gv.sum())} of {len(gv)} graphs valid) in {check_ms:.1f} ms")
     tune = b.autotune(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds) if args.tune else None
-    tiles = tiles_of(tune) if tune else {"lanes": 0, "block_size": 0, "hub_degree": 0, "unroll": 0}
+    tiles = tiles_of(tune) if tune else {"lanes": args.lanes, "block_size": args.block, "hub_degree": args.hub,
+                                         "unroll": args.unroll}
     log(f"tiles {tiles} (autotune {tune})")
 
     def step():

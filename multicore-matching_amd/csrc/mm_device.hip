@@ -1220,23 +1220,93 @@ __global__ void k_unpermute(const int2 *__restrict__ pairs, UnpermOff g, const i
 
 // Rounds each batched graph would execute alone: one plus the last round in
 // which one of its vertices was still active (src/matchgpu.cu:604 loop).
-__global__ void k_graph_rounds(const uint8_t *vround, const int32_t *voff, int32_t n_graphs, int32_t R_run,
+// Three launches: clear the per-graph maxima, fold the vertices' last rounds
+// into them, turn them into round counts.  The fold gives each thread a run
+// of 64 consecutive vertices (graphs are contiguous id ranges, found by a
+// binary search over the graph offsets for the run's first vertex) and posts
+// one atomic max per graph the run touches.
+__global__ void k_graph_clear(int32_t *gmax, int32_t n_graphs)
+{
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_graphs; i += gridDim.x * blockDim.x) gmax[i] = 0;
+}
+
+__global__ void k_graph_fold(const uint8_t *vround, const int32_t *voff, int32_t n_graphs, int32_t *gmax)
+{
+    constexpr int RUN = 64;
+    const int32_t n = voff[n_graphs];
+    const int64_t nruns = ((int64_t)n + RUN - 1) / RUN;
+    // wave-uniform loop: the wave's maxima are combined before the atomics,
+    // which all go to the same few addresses
+    for (int64_t t0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); t0 < nruns;
+         t0 += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = t0 + (threadIdx.x & 63);
+        const bool valid = t < nruns;
+        const int32_t v0 = valid ? (int32_t)(t * RUN) : 0, v1 = valid ? min(n, v0 + RUN) : 0;
+        int32_t lo = 0, hi = n_graphs - 1;  // last graph with voff[g] <= v0
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) >> 1;
+            if (voff[mid] <= v0) lo = mid;
+            else hi = mid - 1;
+        }
+        // the run's bytes: four 16-byte loads (a full run lies inside the array)
+        uint32_t wds[RUN / 4];
+        if (v1 - v0 == RUN) {
+            const uint4 *p = reinterpret_cast<const uint4 *>(vround + v0);
+#pragma unroll
+            for (int q = 0; q < RUN / 16; ++q) {
+                const uint4 c = p[q];
+                wds[4 * q] = c.x;
+                wds[4 * q + 1] = c.y;
+                wds[4 * q + 2] = c.z;
+                wds[4 * q + 3] = c.w;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < RUN / 4; ++q) wds[q] = 0;
+            for (int32_t v = v0; v < v1; ++v) wds[(v - v0) >> 2] |= (uint32_t)vround[v] << (8 * ((v - v0) & 3));
+        }
+        int32_t g = lo, end = voff[g + 1], x = 0;
+        if (end >= v1) {
+            // one graph (the usual case): the byte maximum of the run
+#pragma unroll
+            for (int q = 0; q < RUN / 4; ++q) {
+                const uint32_t w = wds[q];
+                x = max(x, (int32_t)max(max(w & 0xFFu, (w >> 8) & 0xFFu), max((w >> 16) & 0xFFu, w >> 24)));
+            }
+        } else {
+            for (int32_t v = v0; v < v1; ++v) {
+                while (v >= end) {  // (empty graphs are skipped)
+                    if (x) atomicMax(gmax + g, x);
+                    x = 0;
+                    ++g;
+                    end = voff[g + 1];
+                }
+                x = max(x, (int32_t)((wds[(v - v0) >> 2] >> (8 * ((v - v0) & 3))) & 0xFFu));
+            }
+        }
+        // (g, x): the last graph this run touched and its maximum so far
+        const int32_t g0 = __shfl(g, 0);
+        if (__ballot(valid && g != g0) == 0ull) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) x = max(x, __shfl_xor(x, off, 64));
+            if ((threadIdx.x & 63) == 0 && x) atomicMax(gmax + g0, x);
+        } else if (valid && x) {
+            atomicMax(gmax + g, x);
+        }
+    }
+}
+
+__global__ void k_graph_rounds(const int32_t *gmax, const int32_t *voff, int32_t n_graphs, int32_t R_run,
                                int32_t R_max, int32_t maximal, int32_t *out)
 {
-    const int gidx = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (gidx >= n_graphs) return;
-    const int32_t v0 = voff[gidx], v1 = voff[gidx + 1];
-    int32_t x = 0;
-    for (int32_t v = v0 + (threadIdx.x & 63); v < v1; v += 64) x = max(x, (int32_t)vround[v]);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) x = max(x, __shfl_xor(x, off, 64));
-    if ((threadIdx.x & 63) == 0) {
+    for (int32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < n_graphs; g += gridDim.x * blockDim.x) {
+        const int32_t x = gmax[g];
         int32_t rounds;
         if (!maximal) rounds = R_max;
-        else if (v1 == v0) rounds = 1;
+        else if (voff[g + 1] == voff[g]) rounds = 1;
         else if (x == 255) rounds = R_run;
         else rounds = min(R_max, x + 1);
-        out[gidx] = rounds;
+        out[g] = rounds;
     }
 }
 
@@ -2043,8 +2113,14 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     if (graph_rounds && g->d_vround) {
         graph_rounds->assign((size_t)g->n_graphs, 0);
         if (g->n_graphs > 0) {
-            hipLaunchKernelGGL(k_graph_rounds, dim3((g->n_graphs + 3) / 4), dim3(256), 0, g->stream, g->d_vround,
-                               g->d_voff, g->n_graphs, R_run, R_max, maximal, g->d_grounds);
+            const int gb = (int)std::min<int64_t>(((int64_t)g->n_graphs + 255) / 256, 1024);
+            const int fb = (int)std::min<int64_t>(((int64_t)g->n / 64 + 255) / 256 + 1, (int64_t)g->num_cu * 8);
+            int32_t *gmax = g->d_grounds + g->n_graphs;
+            hipLaunchKernelGGL(k_graph_clear, dim3(gb), dim3(256), 0, g->stream, gmax, g->n_graphs);
+            hipLaunchKernelGGL(k_graph_fold, dim3(fb), dim3(256), 0, g->stream, (const uint8_t *)g->d_vround,
+                               (const int32_t *)g->d_voff, g->n_graphs, gmax);
+            hipLaunchKernelGGL(k_graph_rounds, dim3(gb), dim3(256), 0, g->stream, (const int32_t *)gmax,
+                               (const int32_t *)g->d_voff, g->n_graphs, R_run, R_max, maximal, g->d_grounds);
             MM_HIP_TRY(hipMemcpyAsync(graph_rounds->data(), g->d_grounds, sizeof(int32_t) * (size_t)g->n_graphs,
                                       hipMemcpyDeviceToHost, g->stream),
                        "Unable to read batch rounds");
@@ -2337,7 +2413,7 @@ int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_
     g->n_graphs = n_graphs;
     if ((rc = dev_alloc(g, &g->d_voff, (size_t)n_graphs + 1, "batch offsets")) != MM_OK ||
         (rc = dev_alloc(g, &g->d_vround, (size_t)total_n, "batch rounds")) != MM_OK ||
-        (rc = dev_alloc(g, &g->d_grounds, (size_t)n_graphs, "batch rounds")) != MM_OK) {
+        (rc = dev_alloc(g, &g->d_grounds, 2 * (size_t)n_graphs, "batch rounds")) != MM_OK) {
         release(g);
         return rc;
     }
