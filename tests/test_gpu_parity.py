@@ -150,6 +150,23 @@ def test_row_layouts_match_the_oracle(packed):
         g.close()
 
 
+def test_round_one_without_head_copy():
+    """MM_CREATE_NO_HEADS: weighted PUSH rounds then start from work lists
+    (the round-1 select files every vertex) instead of the buckets' id ranges
+    and the head copy; the result is the oracle's either way."""
+    say("no head copy")
+    for csr in (mmamd.gen_rgg(1 << 15, seed=31, weighted=True), mmamd.gen_rmat(13, seed=32, weighted=True)):
+        for heads in (True, False):
+            g = Graph(csr, heads=heads)
+            for lanes in (2, 8):
+                check_against_oracle(csr, g, True, True, lanes=lanes, hub_degree=64)
+                # a fixed 3-round run ends with vertices still active: their colours are part of the result
+                exp = oracle.match(csr, True, False, SEEDS, max_rounds=3)[0]
+                res = g.match(kind=MM_WEIGHTED, maximal=False, max_rounds=3, seeds=SEEDS, lanes=lanes, hub_degree=64)
+                np.testing.assert_array_equal(res.match, exp)
+            g.close()
+
+
 @pytest.mark.parametrize("hub,keep", [(16, False), (256, False), (16, True)])
 def test_rmat_hub_path_bitexact(hub, keep):
     say("rmat scale 15 hub", hub, "keep_order", keep)
