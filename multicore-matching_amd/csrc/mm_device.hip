@@ -65,6 +65,9 @@ struct Dev {
     const int2 *adj;      // weighted rows sorted for matching: {neighbour id (~id when the
                           // weight can never win), weight bits}
     const int2 *head;     // entries 0..3 of every adj row, 4 per vertex (null: none kept)
+    const uint32_t *headc;// the same in 4 bytes per entry: invalid << 31 | id << head_pb | position
+                          // (graphs with 32-bit keys whose ids and positions fit; null otherwise)
+    int32_t head_pb;
     int32_t *out;         // result in the caller's order, written as vertices settle
     const int32_t *vbase; // batches: first caller id of each vertex's graph (null: single graph)
     int32_t *req;
@@ -463,7 +466,7 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
 // with no candidate among those 4 and more entries go to the second pass (G2
 // lanes per row); rows above the hub degree are the hub blocks' work.
 // Weighted rows only (the head copy exists for those).
-template <bool PUSH, int G2>
+template <bool PUSH, int G2, bool HC>
 __device__ void head_scan_range(const Dev &d, int64_t lo, int64_t hi, unsigned long long *resp,
                                 unsigned long long &nnz)
 {
@@ -480,19 +483,31 @@ __device__ void head_scan_range(const Dev &d, int64_t lo, int64_t hi, unsigned l
         const uint8_t sv0 = d.st[v];
         const int2 rg = d.ranges[v];
         const int32_t vloc = d.loc[v];
-        const v4i *hp = reinterpret_cast<const v4i *>(d.head + (size_t)v * 4);
-        const v4i h01 = __builtin_nontemporal_load(hp), h23 = __builtin_nontemporal_load(hp + 1);
-        const int32_t deg0 = rg.y - rg.x;
-        const bool valid = vi < hi && sv0 == ST_BLUE && deg0 <= d.hub_deg;
-        const int32_t deg = valid ? deg0 : 0;
-        const int2 raw[4] = {make_int2(h01.x, h01.y), make_int2(h01.z, h01.w), make_int2(h23.x, h23.y),
-                             make_int2(h23.z, h23.w)};
         int32_t u[4];
         bool ok[4];
         float w[4];
         uint8_t sv[4];
+        if (HC) {
+            const v4i h = __builtin_nontemporal_load(reinterpret_cast<const v4i *>(d.headc + (size_t)v * 4));
+            const uint32_t hw[4] = {(uint32_t)h.x, (uint32_t)h.y, (uint32_t)h.z, (uint32_t)h.w};
+            const uint32_t idmask = (1u << (31 - d.head_pb)) - 1u, pmask = (1u << d.head_pb) - 1u;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) decode_raw<true>(raw[k], u[k], ok[k], w[k]);
+            for (int k = 0; k < 4; ++k) {
+                ok[k] = (hw[k] >> 31) == 0u;
+                u[k] = (int32_t)((hw[k] >> d.head_pb) & idmask);
+                w[k] = __int_as_float((int32_t)(hw[k] & pmask));  // (the key position)
+            }
+        } else {
+            const v4i *hp = reinterpret_cast<const v4i *>(d.head + (size_t)v * 4);
+            const v4i h01 = __builtin_nontemporal_load(hp), h23 = __builtin_nontemporal_load(hp + 1);
+            const int2 raw[4] = {make_int2(h01.x, h01.y), make_int2(h01.z, h01.w), make_int2(h23.x, h23.y),
+                                 make_int2(h23.z, h23.w)};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) decode_raw<true>(raw[k], u[k], ok[k], w[k]);
+        }
+        const int32_t deg0 = rg.y - rg.x;
+        const bool valid = vi < hi && sv0 == ST_BLUE && deg0 <= d.hub_deg;
+        const int32_t deg = valid ? deg0 : 0;
         // (entries past the row end gather vertex 0's slot)
 #pragma unroll
         for (int k = 0; k < 4; ++k) sv[k] = d.st[k < deg ? u[k] : 0];
@@ -708,7 +723,7 @@ __global__ void k_init_select(Dev d)
 // Request phase of round r.  Blocks [0, nb) take bucket b of the blue rows,
 // G lanes per row; blocks [nb, nb + hub_blocks) take the hub rows, one block
 // per row.
-template <bool W, bool A8, bool PUSH, int G, bool HEAD = false>
+template <bool W, bool A8, bool PUSH, int G, bool HEAD = false, bool HC = false>
 __global__ void k_request(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
@@ -722,7 +737,8 @@ __global__ void k_request(Dev d, int r, int hub_blocks)
     }
     const int b = bucket_of_block(d);
     if (HEAD)
-        head_scan_range<PUSH, 8>(d, (int64_t)b * d.bcap, min((int64_t)d.n, (int64_t)(b + 1) * d.bcap), resp, nnz);
+        head_scan_range<PUSH, 8, HC>(d, (int64_t)b * d.bcap, min((int64_t)d.n, (int64_t)(b + 1) * d.bcap), resp,
+                                     nnz);
     else
         first_scan_bucket<W, A8, PUSH, false, G>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap,
                                                  ctrb_ptr(d, r, 0)[b], resp, nnz);
@@ -1347,7 +1363,7 @@ void release(mm_graph *g)
     if (!g) return;
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
-    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_adjp,   g->d_head,  g->d_match, g->d_req,   g->d_st,
+    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_adjp,   g->d_head,   g->d_headc,  g->d_match, g->d_req,   g->d_st,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
                     g->d_unperm, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_lcnt, g->d_llist, g->d_params, g->d_vbase, g->d_chk, g->d_cpart,
@@ -1617,7 +1633,9 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     for (int32_t i = 0; i < n; ++i) max_loc = std::max(max_loc, h_loc[(size_t)i]);
     int locbits = 1;
     while (locbits < 31 && ((int64_t)1 << locbits) <= (int64_t)max_loc) ++locbits;
-    const bool key32 = g->weighted && g->push_ok_weights && n > 0 && locbits < 31 &&
+    // (an edgeless graph keeps no position rows: the device then uses 64-bit keys,
+    // and its heads must hold weights like its rows)
+    const bool key32 = g->weighted && g->push_ok_weights && n > 0 && nnz_dev > 0 && locbits < 31 &&
                        (int64_t)g->max_degree < ((int64_t)1 << (32 - locbits));
     std::vector<int32_t> h_spos;  // sorted position of each row entry in CSR order
     if (key32) h_spos.assign((size_t)nnz_dev, 0);
@@ -1688,9 +1706,31 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
                 for (int32_t k = 0; k < 4 && lo + k < hi; ++k) h_head[4 * (size_t)i + k] = h_scan[(size_t)lo + k];
             }
         });
-        if ((rc = dev_alloc(g, &g->d_head, 4 * nn, "row heads")) != MM_OK) return fail(rc);
-        if (hipMemcpy(g->d_head, h_head.data(), sizeof(int2) * 4 * nn, hipMemcpyHostToDevice) != hipSuccess)
-            return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+        // 4-byte heads when the entries carry key positions and id + position fit
+        // in 31 bits (the scans read only those then), else 8-byte heads
+        int pb = 1, ib = 1;
+        while (pb < 31 && ((int64_t)1 << pb) < (int64_t)g->max_degree) ++pb;
+        while (ib < 31 && ((int64_t)1 << ib) < (int64_t)n) ++ib;
+        if (key32 && ib + pb <= 31) {
+            std::vector<uint32_t> h_headc(4 * nn, 0u);
+            mm::parallel_for_dynamic(n, 4096, [&](int64_t b, int64_t e, unsigned) {
+                for (int64_t i = b; i < e; ++i)
+                    for (int k = 0; k < 4; ++k) {
+                        const int2 x = h_head[4 * (size_t)i + k];
+                        const uint32_t id = (uint32_t)(x.x >= 0 ? x.x : ~x.x);
+                        h_headc[4 * (size_t)i + k] =
+                            (x.x < 0 ? 0x80000000u : 0u) | (id << pb) | ((uint32_t)x.y & ((1u << pb) - 1u));
+                    }
+            });
+            if ((rc = dev_alloc(g, &g->d_headc, 4 * nn, "row heads")) != MM_OK) return fail(rc);
+            if (hipMemcpy(g->d_headc, h_headc.data(), sizeof(uint32_t) * 4 * nn, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+            g->head_pb = pb;
+        } else {
+            if ((rc = dev_alloc(g, &g->d_head, 4 * nn, "row heads")) != MM_OK) return fail(rc);
+            if (hipMemcpy(g->d_head, h_head.data(), sizeof(int2) * 4 * nn, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
+        }
     }
     if (nnz_dev > 0) {
         if (hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz_dev, hipMemcpyHostToDevice) !=
@@ -1819,8 +1859,9 @@ void launch_request(const RunCfg &c, const Dev &d, int r, hipStream_t s)
     if (r == 1 && c.HEAD) {
         const dim3 grid(c.nb + c.hub_blocks), blk(c.block);
         const size_t sm = deferred_smem_bytes(c.block);
-        if (c.PUSH) hipLaunchKernelGGL((k_request<true, true, true, 4, true>), grid, blk, sm, s, d, r, c.hub_blocks);
-        else hipLaunchKernelGGL((k_request<true, true, false, 4, true>), grid, blk, sm, s, d, r, c.hub_blocks);
+        // (HEAD implies PUSH)
+        if (d.headc) hipLaunchKernelGGL((k_request<true, true, true, 4, true, true>), grid, blk, sm, s, d, r, c.hub_blocks);
+        else hipLaunchKernelGGL((k_request<true, true, true, 4, true, false>), grid, blk, sm, s, d, r, c.hub_blocks);
         return;
     }
     if (c.G == 0) {
@@ -1886,9 +1927,12 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     if (o->respond < 0 || o->respond > MM_RESPOND_PUSH) return mm::set_error(MM_ERR_INVALID, "mm_match: bad respond mode");
     c.PUSH = o->respond == MM_RESPOND_PUSH || (o->respond == MM_RESPOND_AUTO && push_ok);
     c.G = G;
-    // The head copy serves the first round of weighted scans with PUSH
-    // responses (not the tiles); that round then runs without lists.
-    c.HEAD = c.A8 && c.PUSH && g->d_head != nullptr && G != 0;
+    // 32-bit keys with the first-candidate scans, when the graph has the rows for them
+    const bool key32 = c.PUSH && c.A8 && G != 0 && g->d_adjp != nullptr;
+    // The head copy (4-byte entries only with 32-bit keys) serves the first
+    // round of weighted PUSH scans (not the tiles); that round then runs
+    // without lists.
+    c.HEAD = c.A8 && c.PUSH && G != 0 && ((key32 && g->d_headc != nullptr) || g->d_head != nullptr);
     c.U = o->entries_per_thread > 0 ? o->entries_per_thread : 4;
     if (c.U != 4 && c.U != 8 && c.U != 16)
         return mm::set_error(MM_ERR_INVALID, "mm_match: entries_per_thread must be 4, 8 or 16");
@@ -1925,9 +1969,11 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.nbrs = g->d_nbrs;
     d.adj = g->d_adj;
     d.head = c.HEAD ? g->d_head : nullptr;
-    // 32-bit keys with the first-candidate scans, when the graph has the rows for them
-    d.key32 = c.PUSH && c.A8 && G != 0 && g->d_adjp != nullptr;
+    d.headc = c.HEAD && key32 ? g->d_headc : nullptr;
+    if (c.HEAD && !d.head && !d.headc) return mm::set_error(MM_ERR_INVALID, "mm_match: no head copy for round 1");
+    d.key32 = key32 ? 1 : 0;
     d.listless1 = c.HEAD ? 1 : 0;
+    d.head_pb = g->head_pb;
     d.locbits = g->locbits;
     if (d.key32) d.adj = g->d_adjp;
     // Vertices write their final values straight into the caller-order result

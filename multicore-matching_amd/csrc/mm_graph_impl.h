@@ -40,6 +40,8 @@ struct mm_graph {
     int locbits = 0;          // bits of local id in a 32-bit proposal key
     int2 *d_head = nullptr;   // weighted: entries 0..3 of each sorted row (adjp when present, else adj),
                               // 4 per vertex (round-1 scans)
+    uint32_t *d_headc = nullptr;  // d_head packed to 4 bytes per entry (invalid << 31 | id << head_pb | position)
+    int head_pb = 0;
     int32_t *d_match = nullptr;
     int32_t *d_req = nullptr;
     uint8_t *d_st = nullptr;
