@@ -2062,8 +2062,12 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
         }
         for (int rr = r0; rr <= r1; ++rr) {
             const int do_select = rr < R_run;
-            MM_LAUNCH(K_REQUEST, launch_request(c, d, rr, g->stream));
-            if (!c.PUSH) MM_LAUNCH(K_RESPOND, launch_respond(c, d, rr, g->stream));
+            // From round 3 on the rows left are few and their first entries
+            // mostly matched: twice the lanes per row (fewer dependent steps).
+            RunCfg cr = c;
+            if (rr >= 3 && c.G > 0) cr.G = std::min(64, 2 * c.G);
+            MM_LAUNCH(K_REQUEST, launch_request(cr, d, rr, g->stream));
+            if (!c.PUSH) MM_LAUNCH(K_RESPOND, launch_respond(cr, d, rr, g->stream));
             const bool ll = rr == 1 && d.listless1;
             if (c.PUSH && ll) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
             else if (c.PUSH) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
