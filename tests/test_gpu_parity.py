@@ -197,7 +197,16 @@ def test_proposal_key_widths():
     w = (rng.integers(0, 1 << 10, size=len(e)) + 1).astype(np.float32) / np.float32(1 << 10)
     wide = CSR.from_edges(n, e, w)
     narrow = mmamd.gen_rgg(1 << 16, seed=22, weighted=True)
-    for csr, bits in ((wide, 64), (narrow, 32)):
+    # 32-bit keys whose ids and positions do not fit the 4-byte head copy
+    # (16 + 16 bits): the head copy keeps 8-byte {id, position} entries
+    m = 1 << 16
+    ring = np.stack([np.arange(m), (np.arange(m) + 1) % m], axis=1)
+    star = np.stack([np.zeros(40000, np.int64), 1 + rng.choice(m - 1, 40000, replace=False)], axis=1)
+    e2 = np.concatenate([ring, star])
+    e2 = np.unique(np.sort(e2[e2[:, 0] != e2[:, 1]], axis=1), axis=0)
+    w2 = (rng.integers(0, 1 << 8, size=len(e2)) + 1).astype(np.float32) / np.float32(1 << 8)
+    mid = CSR.from_edges(m, e2, w2)
+    for csr, bits in ((wide, 64), (narrow, 32), (mid, 32)):
         say("key width", bits)
         g = Graph(csr)
         info = g.info()
