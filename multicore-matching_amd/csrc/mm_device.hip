@@ -4,27 +4,34 @@
 // phases: hash-seeded blue/red select, blue->red request, red response and the
 // match write-back.  Here a round is two or three launches:
 //
-//   request   blue vertices scan their rows (G lanes per row, or a whole block
-//             for rows longer than the hub degree) and pick the heaviest red
-//             neighbour, first in CSR order on ties (gwRequest,
-//             src/matchgpu.cu:360-409) or the first red one (grRequest, 286-327).
-//             PUSH: the proposal is also posted to the red vertex with a 64-bit
-//             atomic max on (weight, -proposer), which replaces the respond scan.
+//   request   blue vertices scan their weight-sorted rows and pick the first
+//             red neighbour in row order: the heaviest, first in CSR order on
+//             ties (gwRequest, src/matchgpu.cu:360-409), or the first red one
+//             of a CSR-order row (grRequest, 286-327).  Round 1 gives each lane
+//             one vertex of its bucket's id range and reads a dense copy of the
+//             rows' first 4 entries; later rounds give G lanes to each entry of
+//             a work list, and rows longer than the hub degree a whole block.
+//             PUSH: the proposal is also posted to the red vertex with one
+//             atomic (32-bit min on the proposer's position in the red
+//             vertex's sorted row, or 64-bit max on (weight, -proposer)),
+//             which replaces the respond scan.
 //   respond   (PULL only) red vertices scan their rows for proposers
 //             (gwRespond/grRespond, src/matchgpu.cu:329-357, 411-451).
 //   match+select  every vertex active in this round resolves its pair
-//             (gMatch, src/matchgpu.cu:260-283), and the ones still active are
-//             recoloured with the next round's seed (gSelect, 184-218) and
-//             appended to the next round's work lists.
+//             (gMatch, src/matchgpu.cu:260-283); settled vertices store their
+//             final value straight into the result in the caller's order, and
+//             the ones still active are recoloured with the next round's seed
+//             (gSelect, 184-218) and appended to the next round's work lists.
 //
-// Only active vertices (match < 2) are ever touched after the first select:
-// they live in four compacted lists per round (blue/red x row length below or
-// above the hub degree), filled with one atomic per wavefront.  Neighbour
-// colours are read from a 1-byte state mirror (st[]) instead of the 4-byte
-// match array, which keeps the random gathers of a 2^22-vertex graph inside
-// L2/MALL.  Every phase writes only the processed vertex's own slots and reads
-// only slots its phase does not write, so the result is independent of the
-// schedule and equals the serial statement of the rounds bit for bit.
+// Only active vertices are touched after round 1: they live in four compacted
+// lists per round (blue/red x row length below or above the hub degree),
+// filled with one LDS atomic per wavefront.  Neighbour colours are read from a
+// 1-byte state mirror (st[]), which keeps the random gathers of a 2^22-vertex
+// graph inside L2/MALL.  Every phase writes only slots that no thread of the
+// same phase reads (the processed vertex's own state, request and list slots;
+// proposal keys only through atomics; result slots, which no phase reads,
+// once per settled vertex), so the result is independent of the schedule and
+// equals the serial statement of the rounds bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
