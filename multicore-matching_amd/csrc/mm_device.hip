@@ -14,7 +14,10 @@
 //             PUSH: the proposal is also posted to the red vertex with one
 //             atomic (32-bit min on the proposer's position in the red
 //             vertex's sorted row, or 64-bit max on (weight, -proposer)),
-//             which replaces the respond scan.
+//             which replaces the respond scan.  In round 1 a proposal to a
+//             vertex of the proposer's own bucket meets the others in the
+//             block's LDS copy of the bucket's keys, stored once at the end
+//             (most rows' heaviest red neighbour lies in their own bucket).
 //   respond   (PULL only) red vertices scan their rows for proposers
 //             (gwRespond/grRespond, src/matchgpu.cu:329-357, 411-451).
 //   match+select  every vertex active in this round resolves its pair
@@ -29,7 +32,8 @@
 // 1-byte state mirror (st[]), which keeps the random gathers of a 2^22-vertex
 // graph inside L2/MALL.  Every phase writes only slots that no thread of the
 // same phase reads (the processed vertex's own state, request and list slots;
-// proposal keys only through atomics; result slots, which no phase reads,
+// proposal keys only through atomics, global or on the block's LDS copy;
+// result slots, which no phase reads,
 // once per settled vertex), so the result is independent of the schedule and
 // equals the serial statement of the rounds bit for bit.
 #include <hip/hip_runtime.h>
@@ -85,6 +89,10 @@ struct Dev {
                                // carry those positions where the weights would be
     int32_t locbits;
     int32_t listless1;         // round 1 runs over the buckets' id ranges: no round-1 lists of short rows
+    int32_t split1;            // round 1 (32-bit keys, head scans): a proposal to a vertex of the
+                               // proposer's own bucket goes to the block's LDS copy of the bucket's
+                               // keys, stored afterwards to bucket_keys(); the others take atomics
+                               // on the round's keys; a key is the smaller of the two
     int4 *lists;               // [2][NLISTS][cap] entries {v, row begin, row end, loc[v]};
                                // classes 0 and 2 split into nb buckets of bcap
     int32_t *ctr;              // [ROUND_SLOTS][CTR_STRIDE] totals; [1],[3] = hub list sizes (atomics)
@@ -282,30 +290,42 @@ __device__ __forceinline__ void clear_key(const Dev &d, unsigned long long *slot
     if (d.key32) reinterpret_cast<uint32_t *>(slot)[v] = 0xFFFFFFFFu;
     else slot[v] = 0ull;
 }
+// Round 1's keys of proposals made inside a bucket (split1): the third n-entry
+// quarter of the key memory, which 32-bit keys leave unused.
+__device__ __forceinline__ uint32_t *bucket_keys(const Dev &d)
+{
+    return reinterpret_cast<uint32_t *>(d.resp) + 2 * (size_t)d.n;
+}
 // Key at vertex v, widened: 32-bit keys come back with the empty key as 0 and
 // the proposer's local id, complemented, in the low half -- the form of the
 // 64-bit keys, which partner_of decodes.
-__device__ __forceinline__ unsigned long long key_at(const Dev &d, const unsigned long long *slot, int32_t v)
+// split: round 1 with split1, whose key is also in bucket_keys().
+__device__ __forceinline__ unsigned long long key_at(const Dev &d, const unsigned long long *slot, int32_t v,
+                                                     bool split = false)
 {
     if (!d.key32) return slot[v];
-    const uint32_t k = reinterpret_cast<const uint32_t *>(slot)[v];
+    uint32_t k = reinterpret_cast<const uint32_t *>(slot)[v];
+    if (split) k = min(k, bucket_keys(d)[v]);
     if (k == 0xFFFFFFFFu) return 0ull;
     return (1ull << 32) | (unsigned long long)(0xFFFFFFFFu - (k & ((1u << d.locbits) - 1u)));
 }
 
 // Writes v's request (or, RESPOND, its accepted proposer) for winner entry
-// (u, w) or none (u < 0).
+// (u, w) or none (u < 0).  s_key (32-bit keys only): the block's LDS keys of
+// the ids [klo, khi), which take the proposals to those ids.
 template <bool W, bool PUSH, bool RESPOND>
 __device__ __forceinline__ void first_write(const Dev &d, int32_t v, int32_t vloc, int32_t u, float w, bool alive,
-                                            unsigned long long *resp)
+                                            unsigned long long *resp, uint32_t *s_key = nullptr, int64_t klo = 0,
+                                            int64_t khi = 0)
 {
     if (u >= 0) {
         d.req[v] = u;
         if (!RESPOND && PUSH) {
-            if (d.key32)  // (w holds the position bits of the winning entry)
-                atomicMin(reinterpret_cast<uint32_t *>(resp) + u,
-                          ((uint32_t)__float_as_int(w) << d.locbits) | (uint32_t)vloc);
-            else
+            if (d.key32) {  // (w holds the position bits of the winning entry)
+                const uint32_t key = ((uint32_t)__float_as_int(w) << d.locbits) | (uint32_t)vloc;
+                if (s_key && u >= klo && u < khi) atomicMin(s_key + (u - klo), key);
+                else atomicMin(reinterpret_cast<uint32_t *>(resp) + u, key);
+            } else
                 atomicMax(resp + u, proposal_key<W>(w, vloc));
         }
     } else {
@@ -359,7 +379,8 @@ extern __shared__ __align__(16) int4 s_deferred[];
 // Second pass over the nd rows deferred to s_def: {v (top bit: alive so
 // far), next entry, row end, loc}, G lanes per row.
 template <bool W, bool A8, bool PUSH, bool RESPOND, int G>
-__device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned long long *resp)
+__device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned long long *resp,
+                              uint32_t *s_key = nullptr, int64_t klo = 0, int64_t khi = 0)
 {
     constexpr int GPW = 64 / G;
     const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
@@ -375,7 +396,7 @@ __device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned 
         float win_w = 1.0f;
         scan_rest<W, A8, RESPOND, G>(d, v, rg, rg.x, !valid, alive, win_u, win_w);
         alive = (__ballot(alive) & gmask) != 0ull;
-        if (valid && gl == 0) first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp);
+        if (valid && gl == 0) first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp, s_key, klo, khi);
     }
 }
 
@@ -402,7 +423,6 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
     const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
     const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (gw * G));
     const int step = (int)(blockDim.x >> 6) * GPW;
-    const int4 none = make_int4(0, 0, 0, 0);
     // list entries are read at clamped indices (always in bounds); validity
     // is decided by the unclamped index
     auto ent_at = [&](int32_t i) { return load_ent(list + min(i, count - 1)); };
@@ -479,9 +499,13 @@ __device__ void head_scan_range(const Dev &d, int64_t lo, int64_t hi, unsigned l
 {
     int4 *s_def = s_deferred;
     const int DEF_CAP = (int)blockDim.x;
+    // split1: the bucket's keys, after the deferred rows
+    uint32_t *s_key = d.split1 ? reinterpret_cast<uint32_t *>(s_deferred + blockDim.x) : nullptr;
     __shared__ int32_t s_ndef;
     if (hi <= lo) return;
     if (threadIdx.x == 0) s_ndef = 0;
+    if (s_key)
+        for (int64_t i = threadIdx.x; i < hi - lo; i += blockDim.x) s_key[i] = 0xFFFFFFFFu;
     __syncthreads();
     typedef int v4i __attribute__((ext_vector_type(4)));
     for (int64_t base = lo + (threadIdx.x & ~63u); base < hi; base += blockDim.x) {
@@ -547,11 +571,16 @@ __device__ void head_scan_range(const Dev &d, int64_t lo, int64_t hi, unsigned l
                 }
             }
         }
-        if (valid && !deferred) first_write<true, PUSH, false>(d, v, vloc, win_u, win_w, alive, resp);
+        if (valid && !deferred) first_write<true, PUSH, false>(d, v, vloc, win_u, win_w, alive, resp, s_key, lo, hi);
         if (valid) nnz += (unsigned long long)deg;
     }
     __syncthreads();
-    deferred_pass<true, true, PUSH, false, G2>(d, s_def, min(s_ndef, DEF_CAP), resp);
+    deferred_pass<true, true, PUSH, false, G2>(d, s_def, min(s_ndef, DEF_CAP), resp, s_key, lo, hi);
+    if (s_key) {
+        __syncthreads();
+        uint32_t *bk = bucket_keys(d);
+        for (int64_t i = threadIdx.x; i < hi - lo; i += blockDim.x) bk[lo + i] = s_key[i];
+    }
 }
 
 // Rows longer than the hub degree: one block per row, blockDim entries per
@@ -1031,23 +1060,24 @@ __device__ __forceinline__ int32_t partner_of(int32_t v, int32_t vloc, int colou
 }
 
 // Loads of partner_of's inputs for vertex v.
+// split: the keys are split between resp and bucket_keys() (round 1, split1).
 template <bool PUSH>
 __device__ __forceinline__ void partner_words(const Dev &d, int32_t v, int colour, const unsigned long long *resp,
-                                              int32_t &a, unsigned long long &k)
+                                              bool split, int32_t &a, unsigned long long &k)
 {
     a = -1;
     k = 0ull;
-    if (PUSH && colour == ST_RED) k = key_at(d, resp, v);
+    if (PUSH && colour == ST_RED) k = key_at(d, resp, v, split);
     else a = d.req[v];
 }
 template <bool PUSH>
-__device__ __forceinline__ void partner_words2(const Dev &d, int colour, const unsigned long long *resp, int32_t a,
-                                               unsigned long long &k, int32_t &pl)
+__device__ __forceinline__ void partner_words2(const Dev &d, int colour, const unsigned long long *resp, bool split,
+                                               int32_t a, unsigned long long &k, int32_t &pl)
 {
     pl = -1;
     if (a < 0) return;
     if (PUSH) {
-        if (colour == ST_BLUE) k = key_at(d, resp, a);
+        if (colour == ST_BLUE) k = key_at(d, resp, a, split);
     } else {
         k = (unsigned long long)(uint32_t)d.req[a];
         pl = d.loc[a];
@@ -1076,6 +1106,7 @@ __global__ void k_match_select(Dev d, int r, int do_select)
     int4 *nred = list_ptr(d, nparity, L_RED_LO) + (size_t)b * d.bcap;
     const unsigned long long *resp = PUSH ? resp_slot(d, parity) : nullptr;
     unsigned long long *nresp = PUSH ? resp_slot(d, nparity) : nullptr;
+    const bool split = PUSH && LL && d.split1;
     const uint32_t seed = do_select ? d.seeds[r] : 0u;
     int32_t n_matched = 0, n_dead = 0;
     unsigned long long deg_sum = 0, hub_deg_sum = 0;
@@ -1114,12 +1145,12 @@ __global__ void k_match_select(Dev d, int r, int do_select)
         for (int q = 0; q < MU; ++q) {
             a[q] = -1;
             k[q] = 0ull;
-            if (col[q] >= 0) partner_words<PUSH>(d, v[q], col[q], resp, a[q], k[q]);
+            if (col[q] >= 0) partner_words<PUSH>(d, v[q], col[q], resp, split, a[q], k[q]);
         }
 #pragma unroll
         for (int q = 0; q < MU; ++q) {
             pl[q] = -1;
-            if (col[q] >= 0) partner_words2<PUSH>(d, col[q], resp, a[q], k[q], pl[q]);
+            if (col[q] >= 0) partner_words2<PUSH>(d, col[q], resp, split, a[q], k[q], pl[q]);
         }
 #pragma unroll
         for (int q = 0; q < MU; ++q) {
@@ -1150,8 +1181,8 @@ __global__ void k_match_select(Dev d, int r, int do_select)
             const int32_t v = ent.x, vloc = ent.w;
             int32_t a, pl;
             unsigned long long k;
-            partner_words<PUSH>(d, v, colour, resp, a, k);
-            partner_words2<PUSH>(d, colour, resp, a, k, pl);
+            partner_words<PUSH>(d, v, colour, resp, split, a, k);
+            partner_words2<PUSH>(d, colour, resp, split, a, k, pl);
             bool dd = false;
             const int32_t partner = partner_of<PUSH>(v, vloc, colour, a, k, pl, dd);
             n_matched += partner >= 0;
@@ -1802,6 +1833,9 @@ int64_t hubs_above(const mm_graph *g, int32_t thr)
 
 // Dynamic LDS of the first-candidate scan kernels: the deferred-row buffer.
 size_t deferred_smem_bytes(int block) { return sizeof(int4) * (size_t)block; }
+// LDS of a round-1 head-scan block: the deferred rows, then (split1) the bucket's keys.
+size_t head_smem_bytes(int block, int bcap, bool split) { return deferred_smem_bytes(block) + (split ? 4 * (size_t)bcap : 0); }
+constexpr size_t HEAD_SMEM_MAX = 60 * 1024;  // (static LDS of the kernel comes on top)
 
 struct RunCfg {
     bool W, A8, PUSH;
@@ -1865,7 +1899,7 @@ void launch_request(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
     if (r == 1 && c.HEAD) {
         const dim3 grid(c.nb + c.hub_blocks), blk(c.block);
-        const size_t sm = deferred_smem_bytes(c.block);
+        const size_t sm = head_smem_bytes(c.block, c.bcap, d.split1 != 0);
         // (HEAD implies PUSH)
         if (d.headc) hipLaunchKernelGGL((k_request<true, true, true, 4, true, true>), grid, blk, sm, s, d, r, c.hub_blocks);
         else hipLaunchKernelGGL((k_request<true, true, true, 4, true, false>), grid, blk, sm, s, d, r, c.hub_blocks);
@@ -1980,6 +2014,8 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     if (c.HEAD && !d.head && !d.headc) return mm::set_error(MM_ERR_INVALID, "mm_match: no head copy for round 1");
     d.key32 = key32 ? 1 : 0;
     d.listless1 = c.HEAD ? 1 : 0;
+    // proposals inside a bucket meet in LDS in round 1 when the bucket's keys fit
+    d.split1 = c.HEAD && key32 && head_smem_bytes(c.block, c.bcap, true) <= HEAD_SMEM_MAX ? 1 : 0;
     d.head_pb = g->head_pb;
     d.locbits = g->locbits;
     if (d.key32) d.adj = g->d_adjp;
