@@ -357,7 +357,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-selection comparison")
     ap.add_argument("--params", default=os.path.join(ROOT, "multicore-matching_amd", "mlp_params.json"),
                     help="network parameters of the learned selection (learn.py train)")
-    ap.add_argument("--keep-order", action="store_true", help="no internal breadth-first renumbering")
+    ap.add_argument("--keep-order", action="store_true", help="no internal locality renumbering")
     ap.add_argument("--packed-rows", action="store_true", help="device rows back to back (no line alignment)")
     ap.add_argument("--no-heads", dest="heads", action="store_false",
                     help="no dense copy of the rows' first entries (round 1 scans the rows)")

@@ -122,8 +122,8 @@ int32_t mm_device_count(void);
 int32_t mm_graph_create(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges,
                         const int32_t *neighbours, const float *weights, mm_graph **out);
 /* As mm_graph_create, with flags.  By default the handle renumbers the vertices
- * internally in breadth-first order, so that neighbours sit close together in
- * memory; results, hash inputs and tie-breaks still use the caller's ids.
+ * internally in a locality order (compact regions of up to 2048 vertices,
+ * each grown breadth-first), so that neighbours sit close together in memory; results, hash inputs and tie-breaks still use the caller's ids.
  * MM_CREATE_KEEP_ORDER keeps the caller's numbering on the device. */
 #define MM_CREATE_KEEP_ORDER 1u
 /* MM_CREATE_PACKED_ROWS stores the device rows back to back.  By default a row

@@ -1492,21 +1492,26 @@ int32_t analyse_csr(mm_graph *g, const int32_t *ranges, const int32_t *nbrs, con
     return MM_OK;
 }
 
-// Breadth-first order of the vertices, components taken in increasing id
-// order: neighbours get nearby positions, so the per-round gathers of vertex
-// state hit the same cache lines.  order[new] = old.  Components never mix,
-// so every batched graph keeps one contiguous block of new ids, in order.
-std::vector<int32_t> bfs_order(int32_t n, const int32_t *ranges, const int32_t *nbrs, const int32_t *vbase)
+// Locality order of the vertices: compact regions of up to REGION vertices,
+// each grown breadth-first, the next one seeded beside the last (components
+// taken in increasing id order).  Neighbours get nearby positions, so the
+// per-round gathers of vertex state hit the same cache lines, and the buckets
+// (ranges of the order) keep most edges inside themselves: on a 2^22-vertex
+// RGG 97% of the edges join two ids of the same 8192-id range, against 76% for
+// one breadth-first sweep, whose long thin levels cross every range.
+// order[new] = old.  Components never mix, so every batched graph keeps one
+// contiguous block of new ids, in order.
+constexpr size_t REGION = 2048;
+std::vector<int32_t> locality_order(int32_t n, const int32_t *ranges, const int32_t *nbrs, const int32_t *vbase)
 {
-    // Breadth-first over the vertices with neighbours (components in order of
-    // their smallest id); vertices without neighbours are then spread evenly
-    // over the order instead of trailing it.  Buckets are ranges of the order,
-    // and an isolated vertex costs a bucket almost nothing, so a graph with
-    // many of them (R-MAT: about half) would otherwise leave the giant
-    // component's buckets with all of the work.  A batch (vbase: first id of
-    // each vertex's graph) is ordered graph by graph, so every graph keeps a
-    // contiguous range of the order.
-    std::vector<int32_t> order((size_t)n), conn, iso;
+    // Regions over the vertices with neighbours; vertices without neighbours
+    // are then spread evenly over the order instead of trailing it.  Buckets
+    // are ranges of the order, and an isolated vertex costs a bucket almost
+    // nothing, so a graph with many of them (R-MAT: about half) would
+    // otherwise leave the giant component's buckets with all of the work.  A
+    // batch (vbase: first id of each vertex's graph) is ordered graph by
+    // graph, so every graph keeps a contiguous range of the order.
+    std::vector<int32_t> order((size_t)n), conn, iso, border;
     std::vector<uint8_t> seen((size_t)n, 0);
     int32_t lo = 0;
     while (lo < n) {
@@ -1517,23 +1522,39 @@ std::vector<int32_t> bfs_order(int32_t n, const int32_t *ranges, const int32_t *
             hi = n;
         conn.clear();
         iso.clear();
-        size_t head = 0;
         for (int32_t s0 = lo; s0 < hi; ++s0) {
             if (seen[(size_t)s0]) continue;
-            seen[(size_t)s0] = 1;
             if (ranges[2 * (size_t)s0 + 1] == ranges[2 * (size_t)s0]) {
+                seen[(size_t)s0] = 1;
                 iso.push_back(s0);  // (a row naming it from elsewhere will not add it again)
                 continue;
             }
-            conn.push_back(s0);
-            while (head < conn.size()) {
-                const int32_t v = conn[head++];
-                for (int32_t j = ranges[2 * (size_t)v]; j < ranges[2 * (size_t)v + 1]; ++j) {
-                    const int32_t u = nbrs[j];
-                    if (!seen[(size_t)u]) {
+            border.assign(1, s0);
+            while (!border.empty()) {
+                const int32_t b = border.back();
+                border.pop_back();
+                if (seen[(size_t)b]) continue;
+                // one region, breadth-first from b until it holds REGION vertices
+                const size_t start = conn.size();
+                size_t head = start;
+                seen[(size_t)b] = 1;
+                conn.push_back(b);
+                while (head < conn.size() && conn.size() - start < REGION) {
+                    const int32_t v = conn[head++];
+                    for (int32_t j = ranges[2 * (size_t)v]; j < ranges[2 * (size_t)v + 1]; ++j) {
+                        const int32_t u = nbrs[j];
+                        if (seen[(size_t)u]) continue;
                         seen[(size_t)u] = 1;
                         conn.push_back(u);
+                        if (conn.size() - start == REGION) break;
                     }
+                }
+                // the unseen neighbours of its last rows seed the next regions,
+                // the most recent first, so the next region grows beside it
+                for (size_t i = head > start ? head - 1 : start; i < conn.size(); ++i) {
+                    const int32_t v = conn[i];
+                    for (int32_t j = ranges[2 * (size_t)v]; j < ranges[2 * (size_t)v + 1]; ++j)
+                        if (!seen[(size_t)nbrs[j]]) border.push_back(nbrs[j]);
                 }
             }
         }
@@ -1552,7 +1573,7 @@ std::vector<int32_t> bfs_order(int32_t n, const int32_t *ranges, const int32_t *
 }
 
 // vbase_old: per ORIGINAL vertex, the first id of its batched graph (null for
-// a single graph).  relabel: renumber the vertices internally (bfs_order).
+// a single graph).  relabel: renumber the vertices internally (locality_order).
 int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *nbrs,
                      const float *w, const int32_t *vbase_old, int relabel, bool align_rows, bool heads,
                      mm_graph **out)
@@ -1625,7 +1646,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     //     candidate in the row.
     g->relabelled = relabel != 0 && n > 1;
     std::vector<int32_t> order;
-    if (g->relabelled) order = bfs_order(n, ranges, nbrs, vbase_old);
+    if (g->relabelled) order = locality_order(n, ranges, nbrs, vbase_old);
     std::vector<int32_t> inv;
     if (g->relabelled) {
         inv.resize(nn);
