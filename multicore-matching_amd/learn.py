@@ -102,7 +102,7 @@ def load_params(path: str = DEFAULT_PARAMS) -> np.ndarray:
     return p
 
 
-def compare(csr, params, seeds=None, graph=None, max_rounds: int = 0) -> dict:
+def compare(csr, params, seeds=None, graph=None, max_rounds: int = 0, reps: int = 5) -> dict:
     """Plain and learned maximal matchings of one graph on the device:
     rounds, matched vertices, weight, device time (result left in HBM), and the first round at
     which the learned run's weight reaches the plain run's final weight."""
@@ -113,23 +113,32 @@ def compare(csr, params, seeds=None, graph=None, max_rounds: int = 0) -> dict:
     g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds, fetch=False)
     g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds, fetch=False)
     # timed runs leave the result on the device (match_ms is the device time of
-    # the rounds and the unpermute); the arrays are copied out afterwards
-    plain = g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds, fetch=False)
+    # the rounds and the unpermute); the arrays are copied out afterwards.  A
+    # maximal run's time includes its host counter read-backs, so each time is
+    # the median of `reps` runs.
+    def median_run(fn):
+        runs = [fn() for _ in range(reps)]
+        runs.sort(key=lambda x: x.match_ms)
+        return runs[len(runs) // 2]
+    plain = median_run(lambda: g.match(kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds,
+                                       fetch=False))
     plain_match = g.copy_match()
-    learned = g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds,
-                              fetch=False)
+    learned = median_run(lambda: g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds,
+                                                 max_rounds=max_rounds, fetch=False))
     learned_match = g.copy_match()
     wp, sp = mmamd.get_weight(csr, plain_match)
     wl, sl = mmamd.get_weight(csr, learned_match)
     reach, reach_ms = None, None
     for r in range(1, learned.rounds + 1):
-        m = g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=False, seeds=seeds, max_rounds=r, fetch=False)
+        g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=False, seeds=seeds, max_rounds=r, fetch=False)
         if mmamd.get_weight(csr, g.copy_match())[0] >= wp:
+            m = median_run(lambda: g.match_learned(params, kind=mmamd.MM_WEIGHTED, maximal=False, seeds=seeds,
+                                                   max_rounds=r, fetch=False))
             reach, reach_ms = r, m.match_ms
             break
     # the same kernels with the plain colouring: the cost of the learned rounds without the network
-    through = g.match_learned(None, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds, max_rounds=max_rounds,
-                              fetch=False)
+    through = median_run(lambda: g.match_learned(None, kind=mmamd.MM_WEIGHTED, maximal=True, seeds=seeds,
+                                                 max_rounds=max_rounds, fetch=False))
     out = {
         "plain": {"rounds": plain.rounds, "matched": sp, "weight": wp, "ms": plain.match_ms,
                   "valid": mmamd.test_matching(csr, plain_match)[0]},
