@@ -89,6 +89,7 @@ struct Dev {
                                // carry those positions where the weights would be
     int32_t locbits;
     int32_t listless1;         // round 1 runs over the buckets' id ranges: no round-1 lists of short rows
+    int32_t no_hubs;           // no row is longer than the hub degree
     int32_t split1;            // round 1 (32-bit keys, head scans): a proposal to a vertex of the
                                // proposer's own bucket goes to the block's LDS copy of the bucket's
                                // keys, stored afterwards to bucket_keys(); the others take atomics
@@ -699,7 +700,10 @@ __global__ void k_init_select(Dev d)
     int4 *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
     int4 *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
     unsigned long long deg_sum = 0, hub_deg_sum = 0;
-    // IU vertices per thread and step, all loads issued before any hash.
+    // IU vertices per thread and step, all loads issued before any hash.  The
+    // row ranges serve the hub test, the round's lists and the degree sums
+    // only: without lists, hubs or sums they are not read.
+    const bool need_rows = !d.listless1 || !d.no_hubs || d.count_nnz;
     constexpr int IU = 4;
     for (int64_t base = lo + (threadIdx.x & ~63); base < hi; base += (int64_t)blockDim.x * IU) {
         int32_t vl[IU];
@@ -711,7 +715,7 @@ __global__ void k_init_select(Dev d)
             rg[q] = make_int2(0, 0);
             if (v < hi) {
                 vl[q] = d.loc[v];
-                rg[q] = d.ranges[v];
+                if (need_rows) rg[q] = d.ranges[v];
             }
         }
 #pragma unroll
@@ -2035,6 +2039,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     if (c.HEAD && !d.head && !d.headc) return mm::set_error(MM_ERR_INVALID, "mm_match: no head copy for round 1");
     d.key32 = key32 ? 1 : 0;
     d.listless1 = c.HEAD ? 1 : 0;
+    d.no_hubs = g->max_degree <= c.hub_deg ? 1 : 0;
     // proposals inside a bucket meet in LDS in round 1 when the bucket's keys fit
     d.split1 = c.HEAD && key32 && head_smem_bytes(c.block, c.bcap, true) <= HEAD_SMEM_MAX ? 1 : 0;
     d.head_pb = g->head_pb;
