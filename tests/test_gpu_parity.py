@@ -216,6 +216,28 @@ def test_proposal_key_widths():
         g.close()
 
 
+def test_round_one_bucket_keys_with_and_without_room():
+    """Round 1 combines the proposals between rows of one bucket in LDS when
+    the bucket's 32-bit keys fit beside the deferred rows, and posts every
+    proposal with a global atomic when they do not: at block 1024 a graph of
+    6 M vertices has 512 buckets of 11719 ids (16 KB of deferred rows + 46 KB
+    of keys, over the 60 KB limit); at block 256 its 2048 buckets fit.  Both
+    give the oracle's result."""
+    say("round-1 bucket keys, 6 M-vertex RGG")
+    csr = mmamd.gen_rgg(6_000_000, seed=31, weighted=True)
+    exp, R, act, mat = oracle.match(csr, True, True, SEEDS, nthreads=0)
+    g = Graph(csr)
+    assert g.info()["key_bits"] == 32
+    for block in (1024, 256):
+        say("block", block)
+        res = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS, lanes=8, block_size=block)
+        assert res.rounds == R
+        mism = np.flatnonzero(res.match != exp)
+        assert mism.size == 0, f"block {block}: {mism.size} mismatches, first at {mism[:5]}"
+        np.testing.assert_array_equal(res.matched, mat)
+    g.close()
+
+
 def test_batch_equals_single_graph_path():
     say("batch of mixed graphs")
     rng = np.random.default_rng(3)
