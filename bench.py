@@ -80,17 +80,20 @@ def kernel_bytes(res: mmamd.MatchResult, rounds: int, push: bool, key_bytes: int
 
 def measured_traffic(workload: str, kernel: str, tiles: dict):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 byte-counter
-    summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy calibration),
-    or None when that summary covers another workload, kernel or tile choice."""
+    summary (FETCH_SIZE and WRITE_SIZE runs scaled by a plain-copy calibration)
+    and the tiles they were measured with: the record of the same tiles, else
+    the record of the same workload and kernel under other tiles (the
+    autotuner's near-equal picks differ from run to run), else (None, None)."""
     try:
         text = open(TRAFFIC_FILE).read()
         recs = json.loads(text.split("```json", 1)[1].split("```", 1)[0])
     except (OSError, IndexError, ValueError):
-        return None
-    for rec in recs if isinstance(recs, list) else [recs]:
-        if rec.get("workload") == workload and rec.get("kernel") == kernel and rec.get("tiles") == tiles:
-            return rec.get("traffic_bytes_per_launch")
-    return None
+        return None, None
+    recs = [r for r in (recs if isinstance(recs, list) else [recs])
+            if r.get("workload") == workload and r.get("kernel") == kernel]
+    for rec in sorted(recs, key=lambda r: r.get("tiles") != tiles):
+        return rec.get("traffic_bytes_per_launch"), rec.get("tiles")
+    return None, None
 
 
 def cpu_baseline(csr, seeds, r99: int, t_budget_s: float = 20.0) -> dict:
@@ -225,9 +228,9 @@ def run_single(args, csr, name: str, workload_desc: str) -> dict:
     launches = prof.kernel_launches[dom]
     achieved = kb[dom] / (kms[dom] * 1e-3) / 1e9
     log(f"profile: kernel_ms={kms} launches={prof.kernel_launches} bytes={kb}")
-    traffic = None
+    traffic, traffic_tiles = None, None
     if not args.keep_order and not args.packed_rows and args.heads and args.respond == 0:
-        traffic = measured_traffic(f"{name}_weighted_t99", dom, tiles)
+        traffic, traffic_tiles = measured_traffic(f"{name}_weighted_t99", dom, tiles)
 
     line = {
         "metric": f"GTEPS to >=99% matched ({workload_desc}, weighted)",
@@ -256,7 +259,7 @@ def run_single(args, csr, name: str, workload_desc: str) -> dict:
         "autotune": tune,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "launches": launches, "avg_launch_us": round(kms[dom] * 1e3 / max(1, launches), 3),
+                     "traffic_tiles": traffic_tiles, "launches": launches, "avg_launch_us": round(kms[dom] * 1e3 / max(1, launches), 3),
                      "bytes_per_launch": kb[dom] / max(1, launches)},
         "round_roofline_frac": round(kb["round_total"] / (t99_event_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
         "kernel_ms": {k: round(v, 5) for k, v in kms.items()},
