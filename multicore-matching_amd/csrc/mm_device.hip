@@ -1092,8 +1092,15 @@ __device__ __forceinline__ void partner_words2(const Dev &d, int colour, const u
 // resolves bucket b of round r's lists (and a share of the hub lists),
 // recolours the vertices still active and files them into bucket b of round
 // r+1's lists.
-template <bool PUSH, bool LL = false>
-__global__ void k_match_select(Dev d, int r, int do_select)
+// TIGHT: held to 64 VGPRs (8 wavefronts per SIMD, a few spilled values), so
+// two blocks of 1024 threads share a CU and all the buckets' blocks of a
+// 2^22-vertex graph run at once: its list rounds 2-5 took 30/20/14/9 instead
+// of 35/23/17/12 us.  Round 1's pass over every id (spills cost it 2 us) and
+// batches of small graphs (4.98 instead of 4.89 ms per batch of 4096) keep
+// their 72-76 registers.
+template <bool PUSH, bool LL = false, bool TIGHT = false>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per_eu(TIGHT ? 8 : 1)))
+void k_match_select(Dev d, int r, int do_select)
 {
     __shared__ int32_t s_cnt[2];
     const int parity = r & 1, nparity = parity ^ 1, b = bucket_of_block(d);
@@ -2138,9 +2145,12 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
             MM_LAUNCH(K_REQUEST, launch_request(cr, d, rr, g->stream));
             if (!c.PUSH) MM_LAUNCH(K_RESPOND, launch_respond(cr, d, rr, g->stream));
             const bool ll = rr == 1 && d.listless1;
+            const bool tight = g->n_graphs == 0;  // (a single graph)
             if (c.PUSH && ll) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
+            else if (c.PUSH && tight) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
             else if (c.PUSH) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
             else if (ll) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
+            else if (tight) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, false, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
             else MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, false>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
         }
         if (maximal) return enq_reduce(r0, std::min(r1 + 1, R_run));
