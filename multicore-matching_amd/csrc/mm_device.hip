@@ -1095,11 +1095,12 @@ __device__ __forceinline__ void partner_words2(const Dev &d, int colour, const u
 // TIGHT: held to 64 VGPRs (8 wavefronts per SIMD, a few spilled values), so
 // two blocks of 1024 threads share a CU and all the buckets' blocks of a
 // 2^22-vertex graph run at once: its list rounds 2-5 took 30/20/14/9 instead
-// of 35/23/17/12 us.  Round 1's pass over every id (spills cost it 2 us) and
-// batches of small graphs (4.98 instead of 4.89 ms per batch of 4096) keep
-// their 72-76 registers.
+// of 35/23/17/12 us.  Batches of small graphs keep 72 registers (4.98 instead
+// of 4.89 ms per batch of 4096 when held).  Round 1's pass over every id takes
+// 2 vertices per thread instead of 4 (56 registers, no spills) and also runs
+// 8 wavefronts per SIMD (61 instead of 64 us).
 template <bool PUSH, bool LL = false, bool TIGHT = false>
-__global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per_eu(TIGHT ? 8 : 1)))
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per_eu((TIGHT || LL) ? 8 : 1)))
 void k_match_select(Dev d, int r, int do_select)
 {
     __shared__ int32_t s_cnt[2];
@@ -1122,7 +1123,7 @@ void k_match_select(Dev d, int r, int do_select)
     int32_t n_matched = 0, n_dead = 0;
     unsigned long long deg_sum = 0, hub_deg_sum = 0;
     // MU vertices per thread: all loads of a step are issued before any store.
-    constexpr int MU = 4;
+    constexpr int MU = LL ? 2 : 4;
     for (int base = threadIdx.x & ~63; base < total; base += blockDim.x * MU) {
         int4 e[MU];
         int32_t v[MU], a[MU], vl[MU], pl[MU];
