@@ -732,7 +732,9 @@ __global__ void k_init_select(Dev d)
                 cls = 2 * c + hub;
                 if (hub) hub_deg_sum += (unsigned long long)deg;
                 else deg_sum += (unsigned long long)deg;
-                if (PUSH && c == ST_RED) clear_key(d, resp_slot(d, parity), v);
+                // every vertex's key is emptied, not only the red ones, so the
+                // stores cover whole lines (a blue vertex's key is never read)
+                if (PUSH) clear_key(d, resp_slot(d, parity), v);
             }
             if (d.listless1) {
                 bucket_count(cls == L_BLUE_LO, s_cnt + 0);
