@@ -1,28 +1,32 @@
 // Graph ingest for the drop-in mtc::Graph: METIS and Matrix Market text
-// readers with the reference's conventions, so real inputs produce the same
-// CSR (and therefore the same matching) as the reference would:
-//   METIS (src/graph.cu:351-511): 1-based ids, optional edge/vertex weights
-//     from the fmt flags, self-loops and repeated neighbours dropped (first
-//     occurrence kept), rows kept in file order; edges = (i, j) with i > j.
-//   Matrix Market (src/graph.cu:86-349): symmetric / skew / Hermitian square
-//     matrices only; entries strictly below the diagonal become edges with
-//     weight |z| = sqrt(re^2 + im^2); each edge is appended to both rows in
-//     edge order.
+// readers, written from the two formats' published descriptions (the METIS
+// manual's graph file format; NIST's "The Matrix Market Exchange Formats").
+// Row order decides ties in the matching rounds, so the CSR each reader builds
+// follows the conventions the reference's readers use for the same files
+// (src/graph.cu:86-349 Matrix Market, 351-470 METIS):
+//   METIS: 1-based ids; rows in file order; a row keeps the first occurrence
+//     of each neighbour and drops self-loops; the edge list holds (i, j, w)
+//     for i > j in row order.
+//   Matrix Market: square matrices with a symmetric, skew-symmetric or
+//     Hermitian pattern; each entry strictly below the diagonal is an edge of
+//     weight |z|; every edge is appended to both of its rows in entry order.
 // Compressed inputs (bin/match.cpp:81-110 reads *.graph.bz2 and *.mtx.bz2
 // through a bzip2 filter) are decompressed while they are read, by libbz2's
 // public streaming API, loaded when the first such file is opened.
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cctype>
+#include <cerrno>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <fstream>
+#include <iostream>
 #include <memory>
 #include <mutex>
-#include <cmath>
-#include <exception>
-#include <iostream>
-#include <set>
-#include <sstream>
 #include <string>
 #include <vector>
 
@@ -36,21 +40,6 @@ namespace mtc
 namespace
 {
 
-[[noreturn]] void fail(const char *msg)
-{
-    cerr << msg << endl;
-    throw exception();
-}
-
-string strip_leading(string line)
-{
-    size_t k = 0;
-    while (k < line.size() && isspace((unsigned char)line[k])) ++k;
-    return line.substr(k);
-}
-
-// bzip2 streaming decompression (libbz2's documented low-level interface:
-// a bz_stream, BZ2_bzDecompressInit / BZ2_bzDecompress / BZ2_bzDecompressEnd).
 struct BzStream {
     char *next_in;
     unsigned int avail_in, total_in_lo32, total_in_hi32;
@@ -184,165 +173,322 @@ std::unique_ptr<std::istream> openGraphInput(const std::string &fileName, std::s
     return std::unique_ptr<std::istream>(in.release());
 }
 
+
+namespace
+{
+
+[[noreturn]] void reject(const string &what)
+{
+    cerr << what << endl;
+    throw exception();
+}
+
+// Splits an istream into lines; counts them for messages.
+class Lines
+{
+    public:
+        explicit Lines(istream &in) : in_(in) {}
+        // Next line (without its newline); false at the end of the input.
+        bool next(string &line)
+        {
+            if (!getline(in_, line)) return false;
+            ++number_;
+            if (!line.empty() && line.back() == '\r') line.pop_back();
+            return true;
+        }
+        long number() const { return number_; }
+
+    private:
+        istream &in_;
+        long number_ = 0;
+};
+
+// Reads whitespace-separated numbers and words off one line.
+class Tokens
+{
+    public:
+        explicit Tokens(const string &line) : p_(line.c_str()), end_(line.c_str() + line.size()) {}
+        bool done()
+        {
+            skip();
+            return p_ == end_;
+        }
+        bool integer(long long &v)
+        {
+            skip();
+            if (p_ == end_) return false;
+            char *stop = nullptr;
+            errno = 0;
+            const long long x = strtoll(p_, &stop, 10);
+            if (stop == p_ || errno == ERANGE || !ends_token(stop)) return false;
+            p_ = stop;
+            v = x;
+            return true;
+        }
+        bool real(double &v)
+        {
+            skip();
+            if (p_ == end_) return false;
+            char *stop = nullptr;
+            const double x = strtod(p_, &stop);
+            if (stop == p_ || !ends_token(stop)) return false;
+            p_ = stop;
+            v = x;
+            return true;
+        }
+        bool word(string &w)
+        {
+            skip();
+            const char *b = p_;
+            while (p_ != end_ && !isspace((unsigned char)*p_)) ++p_;
+            w.assign(b, p_);
+            return !w.empty();
+        }
+
+    private:
+        void skip()
+        {
+            while (p_ != end_ && isspace((unsigned char)*p_)) ++p_;
+        }
+        bool ends_token(const char *q) const { return q == end_ || isspace((unsigned char)*q); }
+        const char *p_, *end_;
+};
+
+bool blank(const string &line)
+{
+    for (char c : line)
+        if (!isspace((unsigned char)c)) return false;
+    return true;
+}
+
+// First character that is not white space ('\0' for a blank line).
+char lead(const string &line)
+{
+    for (char c : line)
+        if (!isspace((unsigned char)c)) return c;
+    return '\0';
+}
+
+string lower(string s)
+{
+    for (char &c : s) c = (char)tolower((unsigned char)c);
+    return s;
+}
+
+// Fills g's rows from its edge list: edge k = (x, y, w) appends y to row x
+// and x to row y, in edge order.
+void rows_from_edges(Graph &g)
+{
+    const int n = g.nrVertices;
+    vector<int64_t> start((size_t)n + 1, 0);
+    for (const Edge &e : g.edges) {
+        ++start[(size_t)e.x + 1];
+        ++start[(size_t)e.y + 1];
+    }
+    for (int i = 0; i < n; ++i) start[(size_t)i + 1] += start[i];
+    if (start[n] > INT32_MAX) reject("Graph too large: more than 2^31 - 1 row entries.");
+    g.neighbours.assign((size_t)start[n], 0);
+    g.neighbourWeights.assign((size_t)start[n], 1.0f);
+    vector<int64_t> fill(start.begin(), start.end() - 1);
+    auto put = [&](int row, int other, float w) {
+        const int64_t k = fill[row]++;
+        g.neighbours[k] = other;
+        g.neighbourWeights[k] = w;
+    };
+    for (const Edge &e : g.edges) {
+        put(e.x, e.y, e.w);
+        put(e.y, e.x, e.w);
+    }
+    g.neighbourRanges.resize(n);
+    for (int i = 0; i < n; ++i) g.neighbourRanges[i] = make_int2((int)start[i], (int)start[(size_t)i + 1]);
+    g.nrEdges = (int)g.edges.size();
+}
+
+}  // namespace
+
+// METIS graph file: a header "n m [fmt [ncon]]" after optional '%' comment
+// lines, then one line per vertex: [size] [ncon vertex weights] and its
+// neighbours (1-based), each followed by a weight when fmt asks for edge
+// weights.  fmt's decimal digits, from the right: edge weights, vertex
+// weights, vertex sizes.  Rows may omit a neighbour's mirror entry; the
+// matchers check the CSR they are given.
 istream &Graph::readMETIS(istream &in)
 {
     clear();
-    int flags = 0;
-    bool have_header = false;
-    while (!have_header && in.good()) {
-        string line;
-        getline(in, line);
-        line = strip_leading(line);
-        if (line.length() < 2 || line[0] == '%') continue;
-        istringstream sline(line);
-        if (!(sline >> nrVertices >> nrEdges)) fail("Unable to read METIS header!");
-        nrVertexWeights = 0;
-        sline >> flags >> nrVertexWeights;
-        have_header = true;
+    Lines lines(in);
+    string line;
+    long long n = -1, m = 0, fmt = 0, ncon = 0;
+    while (lines.next(line)) {
+        if (blank(line) || lead(line) == '%') continue;
+        Tokens t(line);
+        if (!t.integer(n) || !t.integer(m) || n < 0 || n > INT32_MAX)
+            reject("METIS: the header line must start with the vertex and edge counts.");
+        if (!t.done() && !t.integer(fmt)) reject("METIS: unreadable fmt field in the header.");
+        if (!t.done() && !t.integer(ncon)) reject("METIS: unreadable ncon field in the header.");
+        break;
     }
-    const bool weightedEdges = (flags % 10) != 0, weightedVertices = ((flags / 10) % 10) != 0;
-    if (nrVertexWeights != 0 && !weightedVertices) fail("Invalid vertex weight specification!");
-    if (weightedVertices && nrVertexWeights == 0) nrVertexWeights = 1;
-    if (nrVertices < 0) fail("Unable to read METIS header!");
+    if (n < 0) reject("METIS: no header line.");
+    const bool edge_w = fmt % 10 != 0, vertex_w = (fmt / 10) % 10 != 0, vertex_size = (fmt / 100) % 10 != 0;
+    if (ncon != 0 && !vertex_w) reject("METIS: ncon is given but fmt has no vertex weights.");
+    if (vertex_w && ncon == 0) ncon = 1;
+    if (ncon < 0 || ncon > 1024) reject("METIS: ncon out of range.");
 
-    neighbourRanges.assign(nrVertices, make_int2(0, 0));
-    vertexWeights.assign(nrVertexWeights > 0 ? (size_t)nrVertexWeights * nrVertices : 0, 0);
-    bool warnedSelf = false, warnedDuplicates = false;
+    nrVertices = (int)n;
+    nrVertexWeights = (int)ncon;
+    vertexWeights.assign((size_t)ncon * (size_t)n, 0);
+    neighbourRanges.assign((size_t)n, make_int2(0, 0));
+    neighbours.reserve((size_t)max<long long>(0, 2 * m));
+    neighbourWeights.reserve((size_t)max<long long>(0, 2 * m));
+    // last_row[u] = the row that last listed u, for the duplicate test
+    vector<int> last_row((size_t)n, -1);
+    long self_loops = 0, repeats = 0;
     for (int i = 0; i < nrVertices; ++i) {
-        if (!in.good()) fail("Read error!");
-        string line;
-        getline(in, line);
-        istringstream sline(line);
-        for (int k = 0; weightedVertices && k < nrVertexWeights; ++k) sline >> vertexWeights[(size_t)nrVertexWeights * i + k];
-        set<int> seen;
-        neighbourRanges[i].x = (int)neighbours.size();
-        int id;
-        while (sline >> id) {
-            if (id < 1 || id > nrVertices) fail("Invalid neighbour index!");
-            float w = 1.0f;
-            if (weightedEdges) sline >> w;
-            if (id - 1 == i) {
-                if (!warnedSelf) {
-                    warnedSelf = true;
-                    cerr << "Warning, skipping v--v edges!" << endl;
-                }
-                continue;
+        bool got = lines.next(line);
+        while (got && lead(line) == '%') got = lines.next(line);
+        if (!got) {
+            // A file may end without the line of an isolated last vertex.
+            if (i + 1 == nrVertices) {
+                neighbourRanges[i] = make_int2((int)neighbours.size(), (int)neighbours.size());
+                break;
             }
-            if (!seen.insert(id).second) {
-                if (!warnedDuplicates) {
-                    warnedDuplicates = true;
-                    cerr << "Warning, duplicate neighbours!" << endl;
-                }
-                continue;
-            }
-            neighbours.push_back(id - 1);
-            neighbourWeights.push_back(w);
+            reject("METIS: the file ends after " + to_string(i) + " of " + to_string(n) + " vertex lines.");
         }
-        neighbourRanges[i].y = (int)neighbours.size();
+        Tokens t(line);
+        long long x;
+        if (vertex_size && !t.integer(x)) reject("METIS: missing vertex size on line " + to_string(lines.number()) + ".");
+        for (long long k = 0; k < ncon; ++k) {
+            if (!t.integer(x)) reject("METIS: missing vertex weight on line " + to_string(lines.number()) + ".");
+            vertexWeights[(size_t)ncon * i + k] = (int)x;
+        }
+        const int row_begin = (int)neighbours.size();
+        long long id;
+        while (t.integer(id)) {
+            if (id < 1 || id > n)
+                reject("METIS: neighbour " + to_string(id) + " out of range on line " + to_string(lines.number()) + ".");
+            double w = 1.0;
+            if (edge_w && !t.real(w)) reject("METIS: missing edge weight on line " + to_string(lines.number()) + ".");
+            const int u = (int)(id - 1);
+            if (u == i) {
+                ++self_loops;
+                continue;
+            }
+            if (last_row[u] == i) {
+                ++repeats;
+                continue;
+            }
+            last_row[u] = i;
+            neighbours.push_back(u);
+            neighbourWeights.push_back((float)w);
+        }
+        if (!t.done()) reject("METIS: unreadable entry on line " + to_string(lines.number()) + ".");
+        if (neighbours.size() > (size_t)INT32_MAX) reject("METIS: more than 2^31 - 1 row entries.");
+        neighbourRanges[i] = make_int2(row_begin, (int)neighbours.size());
     }
+    if (self_loops) cerr << "Warning: dropped " << self_loops << " self-loop (v--v) entries." << endl;
+    if (repeats) cerr << "Warning: dropped " << repeats << " duplicate neighbour entries." << endl;
     for (int i = 0; i < nrVertices; ++i)
-        for (int j = neighbourRanges[i].x; j < neighbourRanges[i].y; ++j)
-            if (i > neighbours[j]) edges.push_back(Edge(i, neighbours[j], neighbourWeights[j]));
-    if (nrEdges != (int)edges.size()) {
-        cerr << "Warning: Invalid edge count!" << endl;
-        nrEdges = (int)edges.size();
-    }
+        for (int k = neighbourRanges[i].x; k < neighbourRanges[i].y; ++k)
+            if (neighbours[k] < i) edges.push_back(Edge(i, neighbours[k], neighbourWeights[k]));
+    nrEdges = (int)edges.size();
+    if ((long long)nrEdges != m)
+        cerr << "Warning: the header gives " << m << " edges, the rows hold " << nrEdges << "." << endl;
     return in;
 }
 
+// Matrix Market file: the banner "%%MatrixMarket matrix <format> <field>
+// <symmetry>" (case-insensitive), '%' comment lines, a size line
+// ("rows cols entries" for coordinate, "rows cols" for array format) and the
+// entries ("i j [re [im]]", or one value per line for array format).  Only
+// square matrices with a symmetric, skew-symmetric or Hermitian pattern are
+// graphs here.
+//
+// Array format: entry k is placed at row k mod (rows - 1), column
+// k div (rows - 1) (0-based), which is where the reference's reader puts it
+// (src/graph.cu:277-281 wraps the row counter at rows - 1), so both read the
+// same edges from such a file.
 istream &Graph::readMatrixMarket(istream &in)
 {
-    enum { NoStorage, Coordinate, Array } storage = NoStorage;
-    enum { NoData, Real, Complex, Integer, Pattern } data = NoData;
-    enum { NoLayout, General, Symmetric, Skew, Hermitian } layout = NoLayout;
-    bool haveDims = false;
-    int nrRows = 0, nrCols = 0, nrEntries = 0, arrRow = 1, arrCol = 1;
     clear();
-    while (in.good()) {
-        string line;
-        getline(in, line);
-        line = strip_leading(line);
-        if (line.length() < 2) continue;
-        istringstream sline(line);
-        if (line[0] == '%' && line[1] == '%') {
-            string word;
-            while (sline >> word) {
-                if (word == "%%MatrixMarket" || word == "matrix") continue;
-                if (word == "coordinate") storage = Coordinate;
-                else if (word == "array") storage = Array;
-                else if (word == "float" || word == "real" || word == "double") data = Real;
-                else if (word == "complex") data = Complex;
-                else if (word == "int" || word == "integer") data = Integer;
-                else if (word == "pattern") data = Pattern;
-                else if (word == "general") layout = General;
-                else if (word == "symmetric") layout = Symmetric;
-                else if (word == "skew" || word == "skew-symmetric") layout = Skew;
-                else if (word == "hermitean" || word == "Hermitean" || word == "hermitian") layout = Hermitian;
-                else cerr << "Unknown Matrix Market header tag '" << word << "'!" << endl;
+    Lines lines(in);
+    string line;
+    enum class Format { coordinate, array } format = Format::coordinate;
+    enum class Field { real, complex, integer, pattern } field = Field::real;
+    bool banner = false, sized = false;
+    long long rows = 0, cols = 0, declared = 0, seen = 0;
+    while (lines.next(line)) {
+        if (blank(line)) continue;
+        if (lead(line) == '%') {
+            if (banner || line.compare(0, 2, "%%") != 0) continue;  // comment
+            Tokens t(line);
+            string tag, object, fmt_word, field_word, sym_word;
+            t.word(tag);
+            if (lower(tag) != "%%matrixmarket") continue;  // some other %% comment
+            if (!t.word(object) || !t.word(fmt_word) || !t.word(field_word) || !t.word(sym_word))
+                reject("Matrix Market: the banner needs object, format, field and symmetry.");
+            object = lower(object);
+            fmt_word = lower(fmt_word);
+            field_word = lower(field_word);
+            sym_word = lower(sym_word);
+            if (object != "matrix") reject("Matrix Market: object '" + object + "' is not a matrix.");
+            if (fmt_word == "coordinate") format = Format::coordinate;
+            else if (fmt_word == "array") format = Format::array;
+            else reject("Matrix Market: unknown format '" + fmt_word + "'.");
+            if (field_word == "real") field = Field::real;
+            else if (field_word == "complex") field = Field::complex;
+            else if (field_word == "integer") field = Field::integer;
+            else if (field_word == "pattern") field = Field::pattern;
+            else reject("Matrix Market: unknown field '" + field_word + "'.");
+            if (sym_word == "general")
+                reject("Matrix Market: a general (unsymmetric) matrix is not an undirected graph.");
+            if (sym_word != "symmetric" && sym_word != "skew-symmetric" && sym_word != "hermitian")
+                reject("Matrix Market: unknown symmetry '" + sym_word + "'.");
+            if (field == Field::pattern && format == Format::array)
+                reject("Matrix Market: pattern matrices must be in coordinate format.");
+            banner = true;
+            continue;
+        }
+        if (!banner) reject("Matrix Market: data before the %%MatrixMarket banner.");
+        Tokens t(line);
+        if (!sized) {
+            if (!t.integer(rows) || !t.integer(cols)) reject("Matrix Market: unreadable size line.");
+            if (rows != cols) reject("Matrix Market: a symmetric matrix must be square.");
+            if (format == Format::coordinate) {
+                if (!t.integer(declared)) reject("Matrix Market: the size line needs an entry count.");
+            } else {
+                declared = rows * cols;
             }
+            if (rows <= 0 || rows > INT32_MAX || declared <= 0) reject("Matrix Market: empty or oversized matrix.");
+            nrVertices = (int)rows;
+            edges.reserve((size_t)min<long long>(declared, 1LL << 28));
+            sized = true;
             continue;
         }
-        if (line[0] == '%') continue;
-        if (storage == NoStorage || data == NoData || layout == NoLayout) fail("Incomplete Matrix Market header!");
-        if (layout == General) fail("Matrix with unsymmetric nonzero pattern!");
-        if (!haveDims) {
-            sline >> nrRows >> nrCols;
-            if (nrRows != nrCols) fail("Non-square matrix with symmetry flags!");
-            if (storage == Coordinate) sline >> nrEntries;
-            else nrEntries = nrRows * nrCols;
-            if (nrEntries == 0 || nrRows == 0) fail("Invalid Matrix Market dimensions!");
-            nrVertices = nrRows;
-            neighbourRanges.assign(nrVertices, make_int2(0, 0));
-            edges.reserve(nrEntries);
-            haveDims = true;
-            continue;
-        }
-        int row, col;
+        long long r, c;
         double re = 1.0, im = 0.0;
-        if (storage == Coordinate) {
-            sline >> row >> col;
-            if (row < 1 || row > nrRows || col < 1 || col > nrCols) {
-                cerr << "Invalid row and/or column index " << row << ", " << col << "!" << endl;
-                throw exception();
-            }
-            --row;
-            --col;
-            if (data != Pattern) {
-                sline >> re;
-                if (data == Complex) sline >> im;
-            }
+        if (format == Format::coordinate) {
+            if (!t.integer(r) || !t.integer(c)) reject("Matrix Market: unreadable entry on line " + to_string(lines.number()) + ".");
+            if (r < 1 || r > rows || c < 1 || c > cols)
+                reject("Matrix Market: entry (" + to_string(r) + ", " + to_string(c) + ") outside the matrix.");
+            --r;
+            --c;
         } else {
-            row = arrRow - 1;
-            col = arrCol - 1;
-            sline >> re;
-            if (data == Complex) sline >> im;
-            if (++arrRow >= nrRows) {
-                arrRow = 1;
-                arrCol++;
-            }
+            const long long period = rows > 1 ? rows - 1 : 1;
+            r = seen % period;
+            c = seen / period;
         }
-        if (row > col) edges.push_back(Edge(row, col, (float)sqrt(re * re + im * im)));
+        if (field != Field::pattern) {
+            if (!t.real(re)) reject("Matrix Market: missing value on line " + to_string(lines.number()) + ".");
+            if (field == Field::complex && !t.real(im))
+                reject("Matrix Market: missing imaginary part on line " + to_string(lines.number()) + ".");
+        }
+        ++seen;
+        if (r > c) edges.push_back(Edge((int)r, (int)c, (float)sqrt(re * re + im * im)));
     }
-    // Both directions of every edge, appended in edge order.
-    nrEdges = (int)edges.size();
-    vector<int> deg(nrVertices, 0);
-    for (const Edge &e : edges) {
-        ++deg[e.x];
-        ++deg[e.y];
-    }
-    int off = 0;
-    for (int i = 0; i < nrVertices; ++i) {
-        neighbourRanges[i] = make_int2(off, off);
-        off += deg[i];
-    }
-    neighbours.assign(2 * edges.size(), 0);
-    neighbourWeights.assign(2 * edges.size(), 1.0f);
-    for (const Edge &e : edges) {
-        neighbours[neighbourRanges[e.x].y] = e.y;
-        neighbourWeights[neighbourRanges[e.x].y++] = e.w;
-        neighbours[neighbourRanges[e.y].y] = e.x;
-        neighbourWeights[neighbourRanges[e.y].y++] = e.w;
-    }
+    if (!banner) reject("Matrix Market: no %%MatrixMarket banner.");
+    if (!sized) reject("Matrix Market: no size line.");
+    if (format == Format::coordinate && seen != declared)
+        cerr << "Warning: the size line gives " << declared << " entries, the file holds " << seen << "." << endl;
+    rows_from_edges(*this);
     return in;
 }
 

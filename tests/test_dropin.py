@@ -93,3 +93,52 @@ def test_dropin_classes_match_oracle_and_consume_one_seed_per_round(tmp_path, ty
         assert (float(weight), int(size)) == pytest.approx(oracle.get_weight(csr, exp))
         pos += R
     assert lines[-1] == f"next_rand {stream[pos]}"
+
+
+def test_matrix_market_array_storage_and_banner_words(tmp_path):
+    # Array format: entry k goes to row k mod (rows - 1), column k div (rows - 1)
+    # (the reference's reader wraps its row counter at rows - 1); only entries
+    # below the diagonal become edges, so this 3x3 file yields the edge (1, 0)
+    # from its second value and nothing else.
+    p = tmp_path / "a.mtx"
+    p.write_text("%%MatrixMarket matrix array real symmetric\n3 3\n" + "".join(f"{k + 1}.5\n" for k in range(9)))
+    r = run_driver("read", "mtx", p)
+    assert r.returncode == 0, r.stderr
+    n, m, nnz, rows = parse_read(r.stdout)
+    assert (n, m, nnz) == (3, 1, 2)
+    assert rows == [[(1, 2.5)], [(0, 2.5)], []]
+    # banner words are case-insensitive; complex entries weigh |z|; pattern entries weigh 1
+    p = tmp_path / "c.mtx"
+    p.write_text("%%MatrixMarket MATRIX Coordinate COMPLEX Hermitian\n3 3 3\n2 1 3.0 4.0\n3 3 1 1\n3 1 0 -2\n")
+    n, m, nnz, rows = parse_read(run_driver("read", "mtx", p).stdout)
+    assert rows == [[(1, 5.0), (2, 2.0)], [(0, 5.0)], [(0, 2.0)]]
+    p = tmp_path / "p.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate pattern skew-symmetric\n%\n\n3 3 2\n3 2\n2 1\n")
+    n, m, nnz, rows = parse_read(run_driver("read", "mtx", p).stdout)
+    assert rows == [[(1, 1.0)], [(2, 1.0), (0, 1.0)], [(1, 1.0)]]
+    for bad in ("%%MatrixMarket matrix coordinate real unsymmetric\n2 2 1\n2 1 1\n",   # unknown symmetry
+                "%%MatrixMarket vector coordinate real symmetric\n2 2 1\n2 1 1\n",     # not a matrix
+                "%%MatrixMarket matrix coordinate real symmetric\n2 3 1\n2 1 1\n",     # not square
+                "%%MatrixMarket matrix coordinate real symmetric\n2 2 1\n3 1 1\n",     # outside
+                "2 2 1\n2 1 1\n"):                                                     # no banner
+        p.write_text(bad)
+        assert run_driver("read", "mtx", p).returncode != 0, bad
+
+
+def test_metis_vertex_sizes_weights_comments_and_short_files(tmp_path):
+    # fmt 111: vertex size, one vertex weight, edge weights; '%' lines inside
+    # the body are comments; the empty line of the last vertex may be absent
+    p = tmp_path / "g.graph"
+    p.write_text("%c\n\n4 2 111\n7 3 2 0.25\n% a comment\n8 4 1 0.25 3 2\n9 5 2 2\n")
+    r = run_driver("read", "metis", p)
+    assert r.returncode == 0, r.stderr
+    n, m, nnz, rows = parse_read(r.stdout)
+    assert (n, m) == (4, 2)
+    assert rows == [[(1, 0.25)], [(0, 0.25), (2, 2.0)], [(1, 2.0)], []]
+    for bad in ("3 1\n2\n",                   # two vertex lines missing
+                "2 1\n3\n1\n",                # neighbour out of range
+                "2 1 1\n2\n1 1.0\n",          # edge weight missing
+                "2 1 0 2\n2\n1\n",            # ncon without vertex weights
+                "x y\n"):                     # no header
+        p.write_text(bad)
+        assert run_driver("read", "metis", p).returncode != 0, bad
