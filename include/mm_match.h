@@ -195,11 +195,12 @@ int32_t mm_batch_autotune(mm_batch *b, const mm_opts *opts, const mm_tune_opts *
  * testMatching) of the handle's current result, in graph-local values for a
  * batch: size = matched vertices, weight summed in double over both endpoints
  * and halved, and the testMatching counts.  Groups of equal match values are
- * formed per graph; a group's "first two" members are its two smallest
- * internal ids (the reference leaves that order to an unstable sort), so for
- * arrays with groups of three or more, `invalid` may differ from the host's.
- * Match values >= 4 + n are counted as matched to themselves.  The weight of a
- * matched pair is exact for CSRs without repeated neighbours. */
+ * formed per graph, for any value >= 4 (values outside the graph's id range
+ * included, as testMatching's sort groups them); a group's "first two"
+ * members, the pair whose adjacency is checked, are its two smallest caller
+ * ids (the reference leaves that order to an unstable sort).  The weight of a
+ * matched vertex is that of its first CSR neighbour with the same value,
+ * exact for CSRs without repeated neighbours. */
 typedef struct mm_check {
     int64_t under;      /* matched to themselves */
     int64_t multi;      /* matched to more than one vertex */
@@ -213,6 +214,9 @@ int32_t mm_graph_check(mm_graph *g, mm_check *out);
 /* Replaces the handle's current result with match (host, n entries, caller's
  * order) -- to check an array computed elsewhere. */
 int32_t mm_graph_set_match(mm_graph *g, const int32_t *match);
+/* Batch: replaces the batch's current result with match (host, total_n
+ * entries: the graphs' arrays concatenated, graph-local values). */
+int32_t mm_batch_set_match(mm_batch *b, const int32_t *match);
 /* Batch: totals, and per graph (arrays of n_graphs, each may be NULL) the
  * weight, the matched vertices and whether the graph's result is valid. */
 int32_t mm_batch_check(mm_batch *b, mm_check *total, double *graph_weight, int64_t *graph_size,

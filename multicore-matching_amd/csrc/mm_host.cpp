@@ -139,31 +139,59 @@ extern "C" int32_t mm_test_matching(int32_t n, const int32_t *ranges, const int3
 {
     if (n < 0 || (n > 0 && (!ranges || !neighbours || !match)) || !counts)
         return mm::set_error(MM_ERR_INVALID, "mm_test_matching: bad arguments");
-    // Group matched vertices by match value (value - 4 is the smaller endpoint).
-    std::vector<int32_t> cnt((size_t)n, 0), first((size_t)n, -1), second((size_t)n, -1);
+    // testMatching sorts the vertices by value and walks the matched ones
+    // from i1 = the next unvisited vertex, i2 = the one after it: a different
+    // value in i2 makes i1 "matched to itself" and the walk resumes after i2;
+    // otherwise (i1, i2) is checked for adjacency and the rest of the group
+    // is "matched to more than one vertex".  Groups of equal values are thus
+    // entered at their first or (after a single-member step) second member:
+    // entry e' = [size == 1 + e] for the next group.  Groups come from slots
+    // (value 4 + k, k < n, has slot k, so slot order is value order) and, for
+    // larger values, which only invalid arrays hold, from sorted (value,
+    // vertex) pairs; within a group members are in ascending vertex order.
+    std::vector<int32_t> cnt((size_t)n, 0), mem((size_t)n * 3, -1);
+    std::vector<std::pair<int32_t, int32_t>> outside;
     for (int32_t i = 0; i < n; ++i) {
         const int32_t m = match[i];
         if (m < 4) continue;
         const int64_t k = (int64_t)m - 4;
-        if (k >= n) continue;  // counted as matched to itself below
+        if (k >= n) {
+            outside.emplace_back(m, i);
+            continue;
+        }
         const int32_t c = cnt[(size_t)k]++;
-        if (c == 0) first[(size_t)k] = i;
-        else if (c == 1) second[(size_t)k] = i;
+        if (c < 3) mem[(size_t)k * 3 + c] = i;
     }
     int64_t under = 0, multi = 0, invalid = 0, matchable = 0;
-    for (int32_t i = 0; i < n; ++i) if (match[i] >= 4 && (int64_t)match[i] - 4 >= n) ++under;
     auto adjacent = [&](int32_t a, int32_t b) {
         for (int32_t j = ranges[2 * a]; j < ranges[2 * a + 1]; ++j)
             if (neighbours[j] == b) return true;
         return false;
     };
-    for (int32_t k = 0; k < n; ++k) {
-        const int32_t c = cnt[(size_t)k];
-        if (c == 0) continue;
-        if (c == 1) { ++under; continue; }
-        const int32_t a = first[(size_t)k], b = second[(size_t)k];
-        if (!adjacent(a, b) || !adjacent(b, a)) ++invalid;
-        multi += c - 2;
+    int entry = 0;
+    // A group of c members whose first three (ascending) are at[0..2].
+    auto walk = [&](int64_t c, const int32_t *at) {
+        if (entry < c) {
+            if (entry == c - 1) {
+                ++under;
+            } else {
+                const int32_t a = at[entry], b = at[entry + 1];
+                if (!adjacent(a, b) || !adjacent(b, a)) ++invalid;
+                multi += c - entry - 2;
+            }
+        }
+        entry = c == 1 + entry ? 1 : 0;
+    };
+    for (int32_t k = 0; k < n; ++k)
+        if (cnt[(size_t)k]) walk(cnt[(size_t)k], mem.data() + (size_t)k * 3);
+    std::sort(outside.begin(), outside.end());
+    for (size_t a = 0; a < outside.size();) {
+        size_t b = a;
+        while (b < outside.size() && outside[b].first == outside[a].first) ++b;
+        int32_t at[3] = {-1, -1, -1};
+        for (size_t q = a; q < b && q < a + 3; ++q) at[q - a] = outside[q].second;
+        walk((int64_t)(b - a), at);
+        a = b;
     }
     for (int32_t i = 0; i < n; ++i) {
         if (match[i] >= 4) continue;

@@ -199,6 +199,7 @@ SYMBOLS = {
     "mm_graph_check": (_I32, [_P, ctypes.POINTER(_Check)]),
     "mm_graph_set_match": (_I32, [_P, _P]),
     "mm_batch_check": (_I32, [_P, ctypes.POINTER(_Check), _P, _P, _P]),
+    "mm_batch_set_match": (_I32, [_P, _P]),
     "mm_autotune": (_I32, [_P, ctypes.POINTER(_Opts), ctypes.POINTER(_TuneOpts), ctypes.POINTER(_TuneResult)]),
     "mm_batch_autotune": (_I32, [_P, ctypes.POINTER(_Opts), ctypes.POINTER(_TuneOpts),
                                  ctypes.POINTER(_TuneResult)]),
@@ -542,6 +543,9 @@ class Batch:
         nb = np.ascontiguousarray(np.concatenate([g.neighbours for g in graphs]) if graphs
                                   else np.zeros(0, np.int32), dtype=np.int32)
         weighted = bool(graphs) and all(g.weights is not None for g in graphs)
+        if not weighted and any(g.weights is not None for g in graphs):
+            raise ValueError("Batch: either every graph has weights or none has (a weighted matching would "
+                             "otherwise run on unit weights for the graphs that have them)")
         w = np.ascontiguousarray(np.concatenate([g.weights for g in graphs]), dtype=np.float32) if weighted else None
         h = ctypes.c_void_p()
         _check(lib().mm_batch_create(device, len(graphs), _ptr(vo), _ptr(eo), _ptr(ranges), _ptr(nb), _ptr(w),
@@ -579,6 +583,13 @@ class Batch:
                                        ctypes.byref(r)))
         del keep
         return _tune_dict(r)
+
+    def _set_match(self, match):
+        """Replaces the device result with the graphs' arrays concatenated (mm_batch_set_match)."""
+        m = np.ascontiguousarray(match, np.int32)
+        if m.shape != (self.total_n,):
+            raise ValueError("match must hold one value per vertex of the batch")
+        _check(lib().mm_batch_set_match(self._h, _ptr(m)))
 
     def check(self):
         """(totals, per-graph weight, per-graph matched vertices, per-graph valid) on the device."""

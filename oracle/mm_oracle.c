@@ -223,12 +223,13 @@ void orc_get_weight(int32_t n, const int32_t *ranges, const int32_t *nbrs, const
     *size_out = size;
 }
 
-static const int32_t *orc_sort_key;
+static _Thread_local const int32_t *orc_sort_key; /* per thread: callers may check in parallel */
 static int orc_cmp_match(const void *pa, const void *pb)
 {
     /* SortVerticesMatch (src/matchcpu.cu:99-125): unmatched first, then by
-     * match value.  Ties are broken by vertex id here; the counts below do not
-     * depend on the order within a group. */
+     * match value.  Ties are broken by vertex id here (the reference's
+     * std::sort leaves their order open); the order within a group decides
+     * only which pair of a group of three or more is tested for adjacency. */
     const int32_t a = *(const int32_t *)pa, b = *(const int32_t *)pb;
     const int32_t va = orc_sort_key[a], vb = orc_sort_key[b];
     const int ma = ORC_IS_MATCHED(va), mb = ORC_IS_MATCHED(vb);

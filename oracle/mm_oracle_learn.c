@@ -132,7 +132,9 @@ int32_t orc_match_learned(int32_t n, const int32_t *ranges, const int32_t *nbrs,
         const int32_t r = rounds + 1;
         int64_t active = 0;
 
-        /* select */
+        /* select: every vertex reads match[] and writes only its own colour
+         * and features, so the loop runs on all host threads */
+#pragma omp parallel for reduction(+ : active) schedule(dynamic, 4096)
         for (int32_t i = 0; i < n; ++i) {
             if (match[i] >= 2) continue;
             ++active;
@@ -175,6 +177,7 @@ int32_t orc_match_learned(int32_t n, const int32_t *ranges, const int32_t *nbrs,
         keep = active > 0;
 
         /* request (src/matchgpu.cu:286-327 random, 360-409 weighted) */
+#pragma omp parallel for schedule(dynamic, 4096)
         for (int32_t i = 0; i < n; ++i) {
             if (match[i] >= 2) continue;
             if (col[i] != L_BLUE) { req[i] = -1; continue; }
@@ -195,7 +198,9 @@ int32_t orc_match_learned(int32_t n, const int32_t *ranges, const int32_t *nbrs,
             reqw[i] = cand >= 0 ? best : 0.0f;
         }
 
-        /* respond (src/matchgpu.cu:329-357 random, 411-451 weighted) */
+        /* respond (src/matchgpu.cu:329-357 random, 411-451 weighted): red
+         * vertices write their own req[] and read only blue vertices' req[] */
+#pragma omp parallel for schedule(dynamic, 4096)
         for (int32_t i = 0; i < n; ++i) {
             if (match[i] >= 2 || col[i] != L_RED) continue;
             int32_t cand = -1;

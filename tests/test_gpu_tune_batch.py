@@ -61,54 +61,3 @@ def test_batch_of_grids_and_rggs_equals_single_graph_path():
     tuned, _, rounds2 = b.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS, **tiles(t))
     np.testing.assert_array_equal(tuned.match, res.match)
     np.testing.assert_array_equal(rounds2, rounds)
-
-
-def host_check(csr, match):
-    ok, under, multi, invalid, matchable = mmamd.test_matching(csr, match)
-    w, size = mmamd.get_weight(csr, match)
-    return dict(under=under, multi=multi, invalid=invalid, matchable=matchable, size=size, weight=w, valid=int(ok))
-
-
-def assert_same_check(dev, host):
-    for k in ("under", "multi", "invalid", "matchable", "size", "valid"):
-        assert dev[k] == host[k], (k, dev, host)
-    assert abs(dev["weight"] - host["weight"]) <= 1e-9 * max(1.0, host["weight"])
-
-
-@pytest.mark.parametrize("keep", [False, True])
-def test_device_check_equals_host_check(keep):
-    csr = mmamd.gen_rgg(1 << 15, seed=17, weighted=True)
-    g = Graph(csr, keep_order=keep)
-    for maximal in (False, True):
-        res = g.match(kind=MM_WEIGHTED, maximal=maximal, max_rounds=0 if maximal else 3, seeds=SEEDS)
-        assert_same_check(g.check(), host_check(csr, res.match))
-    if not keep:
-        return
-    # corrupted arrays (internal order = caller's order here, so the checked
-    # pair of a larger group is the same on both sides)
-    m = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS).match.copy()
-    matched = np.flatnonzero(m >= 4)
-    rng = np.random.default_rng(3)
-    bad = m.copy()
-    bad[rng.choice(matched, 50, replace=False)] = 1          # partners left alone
-    far = rng.choice(np.flatnonzero(m < 4), 6, replace=False)
-    bad[far[:2]] = 4 + int(far[:2].min())                    # a pair that are not neighbours
-    bad[far[2]] = bad[matched[0]]                            # a third member
-    bad[far[3]] = 4 + csr.n + 7                               # out-of-range value
-    for arr in (m, bad):
-        g.set_match(arr)
-        assert_same_check(g.check(), host_check(csr, arr))
-
-
-def test_batch_device_check_per_graph():
-    specs = workloads.batch_sizes(64, 8, 12, seed=5)
-    graphs = [workloads.batch_graph(*s) for s in specs]
-    b = Batch(graphs)
-    res, per_graph, _ = b.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS)
-    tot, gw, gs, gv = b.check()
-    union, vo = workloads.concat(graphs)
-    assert_same_check(tot, host_check(union, workloads.batch_match_global(res.match, vo)))
-    for k, csr in enumerate(graphs):
-        w, size = mmamd.get_weight(csr, per_graph[k])
-        assert gs[k] == size and gv[k]
-        assert abs(gw[k] - w) <= 1e-9 * max(1.0, w)

@@ -167,3 +167,22 @@ def test_host_match_rejects_bad_input():
     bad = CSR(csr.ranges.copy(), csr.neighbours.copy() + 1000, csr.weights)
     with pytest.raises(mmamd.MatchError):
         mmamd.host_match(bad, 0)
+
+
+def test_library_result_check_equals_the_oracle_on_planted_invalid_arrays():
+    """mm_test_matching / mm_get_weight (the drop-in's GraphMatching::testMatching
+    and getWeight) against the oracle, on valid results and on arrays with every
+    defect testMatching counts, values outside the id range included."""
+    from check_util import plant_invalid
+
+    rng = np.random.default_rng(23)
+    seeds = mmamd.reference_seeds(255)
+    for csr in (mmamd.gen_rgg(3000, seed=41, weighted=True), mmamd.gen_grid(40, 30, seed=42, weighted=True),
+                mmamd.gen_rmat(11, seed=43, weighted=True)):
+        m = oracle.match(csr, True, True, seeds)[0]
+        for arr in (m, plant_invalid(m, csr, rng), plant_invalid(m, csr, rng)):
+            assert mmamd.test_matching(csr, arr) == oracle.test_matching(csr, arr)
+            assert mmamd.get_weight(csr, arr) == oracle.get_weight(csr, arr)
+        bad = plant_invalid(m, csr, rng)
+        exp = oracle.test_matching(csr, bad)
+        assert not exp[0] and exp[1] and exp[2] and exp[3]

@@ -275,6 +275,21 @@ def test_test_matching_known_answers():
     assert oracle.test_matching(csr, np.array([4, 4, 0, 0], np.int32)) == (True, 0, 0, 0, 2)
 
 
+def test_test_matching_walk_resumes_after_the_vertex_it_compared():
+    # testMatching (src/matchcpu.cu:166-213) takes i1 and i2 from the sorted
+    # order; when their values differ it counts i1 as matched to itself and
+    # resumes AFTER i2.  Path 0-1-2-3-4 with values [4, 5, 5, 7, 7]: steps
+    # (v0, v1) under, (v2, v3) under, (v4, end) under -- three, although only
+    # vertex 0 is alone.  Hand-traced from the reference's loop.
+    p5 = CSR.from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4)])
+    assert oracle.test_matching(p5, np.array([4, 5, 5, 7, 7], np.int32))[:4] == (False, 3, 0, 0)
+    # the walk enters the pair {v3, v4} at v4 only after a lone value: [4, 6, 6, 6] on path4
+    # -> (v0, v1) under; then (v2, v3) share 6 and are adjacent: valid pair, no multi
+    assert oracle.test_matching(path4(), np.array([4, 6, 6, 6], np.int32))[:4] == (False, 1, 0, 0)
+    # [5, 5, 5, 9]: (v0, v1) adjacent pair, v2 a third member (multi), (v3, end) under
+    assert oracle.test_matching(path4(), np.array([5, 5, 5, 9], np.int32))[:4] == (False, 1, 1, 0)
+
+
 def brute_max_weight(n, edges, w):
     best = 0.0
     for k in range(len(edges) + 1):
