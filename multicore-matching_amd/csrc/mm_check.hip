@@ -21,24 +21,25 @@
 //     unstable sort);
 //   - the entries e follow from a scan over the slots: each thread composes
 //     the maps e -> e' of its run of slots (resetting to 0 at every graph's
-//     first slot), each block composes its threads' maps, the host chains the
-//     blocks, and a second pass walks every run from its true entry;
+//     first slot), each block composes its threads' maps, one thread chains
+//     the blocks, and a second pass walks every run from its true entry;
 //   - values outside their graph's range, which only invalid arrays hold,
-//     come after all in-range values of their graph; they are listed, sorted
-//     and walked on the host from the state the scan leaves at the graph's
-//     end, and their pairs are tested for adjacency on the device.
+//     come after all in-range values of their graph: they are listed as
+//     (value, caller id) keys, sorted (bitonic network), and walked by one
+//     thread from the state the scan leaves at each graph's end.
 // A separate per-vertex pass counts the unmatched vertices with an unmatched
 // neighbour and sums getWeight's size and weight (the first neighbour in CSR
 // order that shares the value); its per-graph sums are kept per lane while
 // the lane stays in one graph (graphs are contiguous caller-id ranges).
+//
+// Every buffer the kernels use lies in one scratch area laid out by
+// check_layout(); the host side allocates nothing while it runs the kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <climits>
 #include <cstdint>
 #include <cstring>
-#include <tuple>
-#include <vector>
 
 #include "../../include/mm_match.h"
 #include "mm_graph_impl.h"
@@ -49,6 +50,7 @@ namespace {
 constexpr int CBLOCK = 256;
 constexpr int SLOTS_PER_THREAD = 64;
 constexpr int NACC = 6;  // under, multi, invalid, matchable, size, weight (double bits)
+constexpr unsigned long long NO_KEY = ~0ull;
 
 struct CDev {
     int32_t n;
@@ -59,19 +61,22 @@ struct CDev {
     const int32_t *mint;  // the same result in the internal order
     const int32_t *perm;  // caller id -> internal id (null: the same)
     const int32_t *voff;  // batch: vertex offsets of the graphs (caller ids); null: one graph
-    int32_t n_graphs;
+    int32_t n_graphs;     // graphs (1 without voff)
     int32_t *count, *first, *second, *third;  // [n] per slot of in-range values
-    int32_t *oob;              // [0] vertices whose value is outside their graph's range, [1] list length
-    int2 *oob_list;            // {caller id, value} of those vertices
+    int32_t *ctr;              // [0] vertices whose value is outside their graph's range, [1] keys listed
+    unsigned long long *keys;  // (value << 32 | caller id) of those vertices
     int32_t ipb;               // caller ids per block of the per-vertex pass
+    int32_t sgrid;             // blocks of the scan
+    int32_t rows;              // rows of part: blocks of the scan / vertex passes, + 1 for the out-of-range walk
     uint8_t *tmap;             // per thread of the scan: bit e = exit entry for entry e
     uint8_t *bmap;             // per block of the scan
-    const uint8_t *bentry;     // per block of the scan: entry of its first slot
+    uint8_t *bentry;           // per block of the scan: entry of its first slot
     int32_t *gexit;            // per graph: entry state after its last in-range slot
-    unsigned long long *part;  // [blocks][NACC]
-    double *gweight;           // batch: per graph (weight summed over both endpoints)
+    unsigned long long *part;  // [rows][NACC]
+    unsigned long long *tot;   // [NACC] totals
+    double *gweight;           // per graph: weight summed over both endpoints, then halved
     unsigned long long *gsize;
-    int32_t *gbad;
+    int32_t *gbad;             // per graph: 1 when invalid, then turned into "valid"
 };
 
 __device__ __forceinline__ int32_t graph_of(const CDev &d, int32_t c)
@@ -126,7 +131,7 @@ __device__ __forceinline__ float weight_to(const CDev &d, int32_t v, int32_t u)
 }
 
 // Entry of the next group after a group of size s entered at e.
-__device__ __host__ __forceinline__ int next_entry(int32_t s, int e) { return s == 1 + e ? 1 : 0; }
+__device__ __forceinline__ int next_entry(int32_t s, int e) { return s == 1 + e ? 1 : 0; }
 // Maps {0,1} -> {0,1} as two bits (bit e = image of e); compose(f, g) = g after f.
 __device__ __forceinline__ uint8_t compose(uint8_t f, uint8_t g)
 {
@@ -153,6 +158,20 @@ __global__ void k_c_permute(const int2 *pairs, int32_t n, const int32_t *in, int
     }
 }
 
+// Clears the per-graph sums, the counters and the partial sums (the scan and
+// the per-vertex pass fill different columns of different numbers of rows).
+__global__ void k_c_zero(CDev d)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.n_graphs; i += gridDim.x * blockDim.x) {
+        d.gweight[i] = 0.0;
+        d.gsize[i] = 0ull;
+        d.gbad[i] = 0;
+        d.gexit[i] = 0;
+    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.rows * NACC; i += gridDim.x * blockDim.x) d.part[i] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x < 4) d.ctr[threadIdx.x] = 0;
+}
+
 // Empties the slots and counts the vertices whose value lies outside their
 // graph's id range.
 __global__ void k_c_init(CDev d)
@@ -167,7 +186,7 @@ __global__ void k_c_init(CDev d)
         if (m >= 4) oob += slot_of(d, graph_of(d, (int32_t)i), m) < 0;
     }
     for (int o = 32; o > 0; o >>= 1) oob += __shfl_xor(oob, o);
-    if ((threadIdx.x & 63) == 0 && oob) atomicAdd(d.oob, oob);
+    if ((threadIdx.x & 63) == 0 && oob) atomicAdd(d.ctr, oob);
 }
 
 // pass 0: size and smallest member of every group; lists the out-of-range
@@ -180,7 +199,8 @@ __global__ void k_c_group(CDev d, int pass)
         if (m < 4) continue;
         const int64_t s = slot_of(d, graph_of(d, c), m);
         if (s < 0) {
-            if (pass == 0 && d.oob_list) d.oob_list[atomicAdd(d.oob + 1, 1)] = make_int2(c, m);
+            if (pass == 0 && d.keys)
+                d.keys[atomicAdd(d.ctr + 1, 1)] = ((unsigned long long)(uint32_t)m << 32) | (uint32_t)c;
             continue;
         }
         if (pass == 0) {
@@ -190,6 +210,29 @@ __global__ void k_c_group(CDev d, int pass)
             if (d.first[s] != c) atomicMin(d.second + s, c);
         } else if (d.first[s] != c && d.second[s] != c) {
             atomicMin(d.third + s, c);
+        }
+    }
+}
+
+// Keys [from, to) get the largest key (padding of the sort).
+__global__ void k_c_pad(unsigned long long *keys, int64_t from, int64_t to)
+{
+    for (int64_t i = from + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < to; i += (int64_t)gridDim.x * blockDim.x)
+        keys[i] = NO_KEY;
+}
+
+// One compare-exchange step (k, j) of a bitonic sorting network over the
+// power-of-two count of keys.
+__global__ void k_c_bitonic(unsigned long long *keys, int64_t count, int64_t k, int64_t j)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t l = i ^ j;
+        if (l <= i) continue;
+        const unsigned long long a = keys[i], b = keys[l];
+        const bool up = (i & k) == 0;
+        if (up ? a > b : a < b) {
+            keys[i] = b;
+            keys[l] = a;
         }
     }
 }
@@ -233,6 +276,17 @@ __global__ __launch_bounds__(CBLOCK) void k_c_scan_maps(CDev d)
         uint8_t b = MAP_ID;
         for (int t = 0; t < CBLOCK; ++t) b = compose(b, maps[t]);
         d.bmap[blockIdx.x] = b;
+    }
+}
+
+// One thread chains the blocks' maps: the entry of every block's first slot.
+__global__ void k_c_chain(CDev d)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int e = 0;
+    for (int b = 0; b < d.sgrid; ++b) {
+        d.bentry[b] = (uint8_t)e;
+        e = (d.bmap[b] >> e) & 1;
     }
 }
 
@@ -311,7 +365,7 @@ __global__ __launch_bounds__(CBLOCK) void k_c_scan_walk(CDev d)
                         }
                     }
                 }
-                if (bad && d.gbad) atomicMax(d.gbad + gi, 1);
+                if (bad) atomicMax(d.gbad + gi, 1);
                 e = next_entry(cnt, e);
             }
             if (s + 1 == gnext) d.gexit[gi] = e;
@@ -322,6 +376,48 @@ __global__ __launch_bounds__(CBLOCK) void k_c_scan_walk(CDev d)
         const unsigned long long v = block_sum_u64(tot[k]);
         if (threadIdx.x == 0) d.part[(size_t)blockIdx.x * NACC + k] = v;
     }
+}
+
+// One thread walks the sorted out-of-range keys, graph by graph from the
+// state each graph's in-range groups left (gexit, advanced in place).  Keys
+// of one value are ordered by caller id, so the keys of one graph and value
+// are consecutive, and each graph's values come in ascending order.
+__global__ void k_c_oob_walk(CDev d, int64_t count)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    unsigned long long under = 0, multi = 0, invalid = 0;
+    int64_t a = 0;
+    while (a < count) {
+        const uint32_t m = (uint32_t)(d.keys[a] >> 32);
+        const int32_t gi = graph_of(d, (int32_t)(uint32_t)d.keys[a]);
+        int64_t b = a + 1;
+        while (b < count && (uint32_t)(d.keys[b] >> 32) == m && graph_of(d, (int32_t)(uint32_t)d.keys[b]) == gi) ++b;
+        const int64_t cnt = b - a;
+        const int e = d.gexit[gi];
+        int bad = 0;
+        if (e < cnt) {
+            if (e == cnt - 1) {
+                ++under;
+                bad = 1;
+            } else {
+                if (!adjacent_callers(d, (int32_t)(uint32_t)d.keys[a + e], (int32_t)(uint32_t)d.keys[a + e + 1])) {
+                    ++invalid;
+                    bad = 1;
+                }
+                if (cnt - e - 2 > 0) {
+                    multi += (unsigned long long)(cnt - e - 2);
+                    bad = 1;
+                }
+            }
+        }
+        if (bad) d.gbad[gi] = 1;
+        d.gexit[gi] = cnt == 1 + e ? 1 : 0;
+        a = b;
+    }
+    unsigned long long *q = d.part + (size_t)(d.rows - 1) * NACC;
+    q[0] = under;
+    q[1] = multi;
+    q[2] = invalid;
 }
 
 // One lane's sums for the graph it is in.
@@ -420,21 +516,31 @@ __global__ __launch_bounds__(CBLOCK) void k_c_vertices(CDev d)
     }
 }
 
-// Adjacency of caller-id pairs (out-of-range groups): ok[k] = 1 when the two
-// are each other's neighbours.
-__global__ void k_c_pairs(CDev d, const int2 *pairs, int32_t count, int32_t *ok)
+// Totals over the rows of partial sums (one block; fixed order), and the
+// per-graph outputs: weight halved, "bad" turned into "valid".
+__global__ __launch_bounds__(CBLOCK) void k_c_total(CDev d)
 {
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x)
-        ok[k] = adjacent_callers(d, pairs[k].x, pairs[k].y) ? 1 : 0;
-}
-
-__global__ void k_c_zero(double *gw, unsigned long long *gs, int32_t *gb, int32_t *ge, int32_t count)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
-        if (gw) gw[i] = 0.0;
-        if (gs) gs[i] = 0ull;
-        if (gb) gb[i] = 0;
-        ge[i] = 0;
+    unsigned long long acc[5] = {0, 0, 0, 0, 0};
+    double w = 0.0;
+    for (int r = threadIdx.x; r < d.rows; r += CBLOCK) {
+        const unsigned long long *q = d.part + (size_t)r * NACC;
+        for (int k = 0; k < 5; ++k) acc[k] += q[k];
+        double x;
+        std::memcpy(&x, &q[5], sizeof(x));
+        w += x;
+    }
+    for (int k = 0; k < 5; ++k) {
+        const unsigned long long s = block_sum_u64(acc[k]);
+        if (threadIdx.x == 0) d.tot[k] = s;
+    }
+    const double ws = block_sum_f64(w);
+    if (threadIdx.x == 0) {
+        const double half = ws / 2.0;
+        std::memcpy(&d.tot[5], &half, sizeof(half));
+    }
+    for (int i = threadIdx.x; i < d.n_graphs; i += CBLOCK) {
+        d.gweight[i] = d.gweight[i] / 2.0;
+        d.gbad[i] = d.gbad[i] ? 0 : 1;
     }
 }
 
@@ -446,24 +552,199 @@ __global__ void k_c_zero(double *gw, unsigned long long *gs, int32_t *gb, int32_
                                   what, hipGetErrorString(e_));                                    \
     } while (0)
 
-// Device memory freed when the check returns.
-struct Tmp {
-    std::vector<void *> ptrs;
-    ~Tmp()
-    {
-        for (void *q : ptrs) (void)hipFree(q);
-    }
-    hipError_t get(void **p, size_t bytes)
-    {
-        *p = nullptr;
-        const hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 1);
-        if (e == hipSuccess) ptrs.push_back(*p);
-        return e;
-    }
+// Scratch of one check: byte offsets into one area, each 256-byte aligned.
+struct Layout {
+    int grid, sgrid, vgrid, rows;
+    int32_t ipb;
+    size_t slots, perm, ctr, tot, part, maps, gexit, gweight, gsize, gbad, keys, bytes;
 };
 
-int32_t run_check(mm_graph *g, const int32_t *voff, int32_t n_graphs, mm_check *out, double *gw_out,
-                  int64_t *gs_out, int32_t *gv_out)
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// key_cap: out-of-range keys to make room for at the end (0: none).
+Layout check_layout(const mm_graph *g, int32_t ng, size_t key_cap)
+{
+    Layout L;
+    const int64_t n = g->n;
+    L.grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + CBLOCK - 1) / CBLOCK, (int64_t)g->num_cu * 8));
+    L.sgrid = (int)std::max<int64_t>(1, (n + (int64_t)CBLOCK * SLOTS_PER_THREAD - 1) / ((int64_t)CBLOCK * SLOTS_PER_THREAD));
+    L.ipb = (int32_t)(((n + L.grid - 1) / L.grid + CBLOCK - 1) / CBLOCK * CBLOCK);
+    if (L.ipb < CBLOCK) L.ipb = CBLOCK;
+    L.vgrid = (int)std::max<int64_t>(1, (n + L.ipb - 1) / L.ipb);
+    L.rows = std::max(L.sgrid, L.vgrid) + 1;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = align256(off + bytes);
+        return at;
+    };
+    L.slots = take(sizeof(int32_t) * 4 * (size_t)n);
+    L.perm = take(sizeof(int32_t) * (size_t)n);
+    L.ctr = take(sizeof(int32_t) * 4);
+    L.tot = take(sizeof(unsigned long long) * NACC);
+    L.part = take(sizeof(unsigned long long) * NACC * (size_t)L.rows);
+    L.maps = take((size_t)L.sgrid * (CBLOCK + 2));
+    L.gexit = take(sizeof(int32_t) * (size_t)ng);
+    L.gweight = take(sizeof(double) * (size_t)ng);
+    L.gsize = take(sizeof(unsigned long long) * (size_t)ng);
+    L.gbad = take(sizeof(int32_t) * (size_t)ng);
+    L.keys = take(sizeof(unsigned long long) * key_cap);
+    L.bytes = off;
+    return L;
+}
+
+size_t pow2_at_least(size_t x)
+{
+    size_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// Where a check's scratch comes from.  base: the area of check_layout(g, ng,
+// 0) (or, when key_cap > 0, of check_layout(g, ng, key_cap) with room for
+// that many keys at L.keys).  perm_ready: base + L.perm already holds the
+// caller -> internal map of g.  grow_keys: called with the out-of-range count
+// when key_cap is too small; returns the keys' address, or null after setting
+// an error.
+struct Scratch {
+    char *base = nullptr;
+    size_t key_cap = 0;
+    bool *perm_ready = nullptr;
+    mm_graph *g = nullptr;
+    unsigned long long *(*grow_keys)(mm_graph *, size_t) = nullptr;
+};
+
+// Runs the check with its scratch in s; per-graph outputs (may be NULL) go
+// straight from the device to the caller.
+int32_t run_check(mm_graph *g, const int32_t *voff, int32_t n_graphs, const Scratch &s, mm_check *out,
+                  double *gw_out, int64_t *gs_out, int32_t *gv_out)
+{
+    const bool batch = voff && n_graphs > 0;
+    const int32_t ng = batch ? n_graphs : 1;
+    const Layout L = check_layout(g, ng, 0);
+    const int64_t n = g->n;
+    char *base = s.base;
+    int32_t *perm = (int32_t *)(base + L.perm);
+    const int pgrid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)g->num_cu * 8);
+    if (g->relabelled && !*s.perm_ready) {
+        hipLaunchKernelGGL(k_c_perm, dim3(pgrid), dim3(256), 0, g->stream, (const int2 *)g->d_unperm, g->n, perm);
+        CK_TRY(hipGetLastError(), "k_c_perm");
+        *s.perm_ready = true;
+    }
+    if (g->internal_stale) {
+        // The last plain run wrote only the caller-order result: bring the
+        // internal-order array the neighbour tests read up to date.
+        hipLaunchKernelGGL(k_c_permute, dim3(pgrid), dim3(256), 0, g->stream, (const int2 *)g->d_unperm, g->n,
+                           (const int32_t *)g->d_match_out, g->d_match);
+        CK_TRY(hipGetLastError(), "k_c_permute");
+        g->internal_stale = false;
+    }
+    CDev d;
+    std::memset(&d, 0, sizeof(d));
+    d.n = g->n;
+    d.ranges = g->d_ranges;
+    d.nbrs = g->d_nbrs;
+    d.adj = g->weighted ? g->d_adj : nullptr;
+    d.res = g->relabelled ? g->d_match_out : g->d_match;
+    d.mint = g->d_match;
+    d.perm = g->relabelled ? perm : nullptr;
+    d.voff = batch ? g->d_voff : nullptr;
+    d.n_graphs = ng;
+    int32_t *slots = (int32_t *)(base + L.slots);
+    d.count = slots;
+    d.first = slots + n;
+    d.second = slots + 2 * (size_t)n;
+    d.third = slots + 3 * (size_t)n;
+    d.ctr = (int32_t *)(base + L.ctr);
+    d.ipb = L.ipb;
+    d.sgrid = L.sgrid;
+    d.rows = L.rows;
+    d.tmap = (uint8_t *)(base + L.maps);
+    d.bmap = d.tmap + (size_t)L.sgrid * CBLOCK;
+    d.bentry = d.bmap + L.sgrid;
+    d.gexit = (int32_t *)(base + L.gexit);
+    d.part = (unsigned long long *)(base + L.part);
+    d.tot = (unsigned long long *)(base + L.tot);
+    d.gweight = (double *)(base + L.gweight);
+    d.gsize = (unsigned long long *)(base + L.gsize);
+    d.gbad = (int32_t *)(base + L.gbad);
+    hipLaunchKernelGGL(k_c_zero, dim3(std::max(1, std::min(1024, (ng + 255) / 256))), dim3(256), 0, g->stream, d);
+    hipLaunchKernelGGL(k_c_init, dim3(L.grid), dim3(CBLOCK), 0, g->stream, d);
+    CK_TRY(hipGetLastError(), "k_c_init");
+    int32_t oob = 0;
+    CK_TRY(hipMemcpyAsync(&oob, d.ctr, sizeof(oob), hipMemcpyDeviceToHost, g->stream), "check counters");
+    CK_TRY(hipStreamSynchronize(g->stream), "check counters");
+    size_t sorted = 0;
+    if (oob > 0) {
+        sorted = pow2_at_least((size_t)oob);
+        if (sorted <= s.key_cap) {
+            d.keys = (unsigned long long *)(base + L.keys);
+        } else {
+            if (!s.grow_keys)
+                return mm::set_error(MM_ERR_INVALID, "mm_graph_check: no room for the out-of-range values' keys");
+            d.keys = s.grow_keys(s.g, sorted);
+            if (!d.keys) return MM_ERR_NOMEM;
+        }
+    }
+    for (int pass = 0; pass < 3; ++pass)
+        hipLaunchKernelGGL(k_c_group, dim3(L.grid), dim3(CBLOCK), 0, g->stream, d, pass);
+    hipLaunchKernelGGL(k_c_scan_maps, dim3(L.sgrid), dim3(CBLOCK), 0, g->stream, d);
+    hipLaunchKernelGGL(k_c_chain, dim3(1), dim3(64), 0, g->stream, d);
+    hipLaunchKernelGGL(k_c_scan_walk, dim3(L.sgrid), dim3(CBLOCK), 0, g->stream, d);
+    hipLaunchKernelGGL(k_c_vertices, dim3(L.vgrid), dim3(CBLOCK), 0, g->stream, d);
+    if (oob > 0) {
+        const int sgrid_keys = (int)std::min<size_t>(1024, (sorted + 255) / 256);
+        hipLaunchKernelGGL(k_c_pad, dim3(sgrid_keys), dim3(256), 0, g->stream, d.keys, (int64_t)oob, (int64_t)sorted);
+        for (int64_t k = 2; k <= (int64_t)sorted; k <<= 1)
+            for (int64_t j = k >> 1; j > 0; j >>= 1)
+                hipLaunchKernelGGL(k_c_bitonic, dim3(sgrid_keys), dim3(256), 0, g->stream, d.keys, (int64_t)sorted, k, j);
+        hipLaunchKernelGGL(k_c_oob_walk, dim3(1), dim3(64), 0, g->stream, d, (int64_t)oob);
+    }
+    hipLaunchKernelGGL(k_c_total, dim3(1), dim3(CBLOCK), 0, g->stream, d);
+    CK_TRY(hipGetLastError(), "result check kernels");
+    unsigned long long tot[NACC];
+    CK_TRY(hipMemcpyAsync(tot, d.tot, sizeof(tot), hipMemcpyDeviceToHost, g->stream), "Unable to read check sums");
+    if (batch) {
+        if (gw_out)
+            CK_TRY(hipMemcpyAsync(gw_out, d.gweight, sizeof(double) * (size_t)ng, hipMemcpyDeviceToHost, g->stream),
+                   "Unable to read check sums");
+        if (gs_out)
+            CK_TRY(hipMemcpyAsync(gs_out, d.gsize, sizeof(int64_t) * (size_t)ng, hipMemcpyDeviceToHost, g->stream),
+                   "Unable to read check sums");
+        if (gv_out)
+            CK_TRY(hipMemcpyAsync(gv_out, d.gbad, sizeof(int32_t) * (size_t)ng, hipMemcpyDeviceToHost, g->stream),
+                   "Unable to read check sums");
+    }
+    CK_TRY(hipStreamSynchronize(g->stream), "result check");
+    out->under = (int64_t)tot[0];
+    out->multi = (int64_t)tot[1];
+    out->invalid = (int64_t)tot[2];
+    out->matchable = (int64_t)tot[3];
+    out->size = (int64_t)tot[4];
+    std::memcpy(&out->weight, &tot[5], sizeof(double));
+    out->valid = out->under == 0 && out->multi == 0 && out->invalid == 0;
+    return MM_OK;
+}
+
+// Handle-owned scratch: grown on demand, kept for the next check.
+unsigned long long *grow_handle_keys(mm_graph *g, size_t cap)
+{
+    if (g->chk_key_cap >= cap) return g->d_chk_keys;
+    if (g->d_chk_keys) {
+        (void)hipFree(g->d_chk_keys);
+        g->device_bytes -= (int64_t)(sizeof(unsigned long long) * g->chk_key_cap);
+        g->d_chk_keys = nullptr;
+        g->chk_key_cap = 0;
+    }
+    void *p = nullptr;
+    if (mm_graph_alloc(g, &p, sizeof(unsigned long long) * cap, "the check's sort keys") != MM_OK) return nullptr;
+    g->d_chk_keys = (unsigned long long *)p;
+    g->chk_key_cap = cap;
+    return g->d_chk_keys;
+}
+
+int32_t check_with_handle_scratch(mm_graph *g, const int32_t *voff, int32_t n_graphs, mm_check *out,
+                                  double *gw_out, int64_t *gs_out, int32_t *gv_out)
 {
     if (!out) return mm::set_error(MM_ERR_INVALID, "mm_graph_check: out is NULL");
     std::memset(out, 0, sizeof(*out));
@@ -475,213 +756,27 @@ int32_t run_check(mm_graph *g, const int32_t *voff, int32_t n_graphs, mm_check *
     }
     if (g->n == 0) return MM_OK;
     CK_TRY(hipSetDevice(g->device), "hipSetDevice");
-    const int64_t n = g->n;
-    const int grid = (int)std::min<int64_t>((n + CBLOCK - 1) / CBLOCK, (int64_t)g->num_cu * 8);
-    const int sgrid = (int)((n + (int64_t)CBLOCK * SLOTS_PER_THREAD - 1) / ((int64_t)CBLOCK * SLOTS_PER_THREAD));
-    const int pblocks = std::max(grid, sgrid);
-    int32_t rc;
-    void *p = nullptr;
-    if (!g->d_chk) {
-        if ((rc = mm_graph_alloc(g, &p, sizeof(int32_t) * 5 * (size_t)n + 2 * sizeof(int32_t), "check scratch")) != MM_OK)
-            return rc;
-        g->d_chk = (int32_t *)p;
+    const Layout L = check_layout(g, voff && n_graphs > 0 ? n_graphs : 1, 0);
+    if (g->chk_bytes < L.bytes) {
+        if (g->d_chk) {
+            (void)hipFree(g->d_chk);
+            g->device_bytes -= (int64_t)g->chk_bytes;
+            g->d_chk = nullptr;
+            g->chk_bytes = 0;
+        }
+        void *p = nullptr;
+        const int32_t rc = mm_graph_alloc(g, &p, L.bytes, "check scratch");
+        if (rc != MM_OK) return rc;
+        g->d_chk = p;
+        g->chk_bytes = L.bytes;
         g->chk_perm = false;
     }
-    Tmp tmp;
-    CK_TRY(tmp.get(&p, sizeof(unsigned long long) * NACC * (size_t)pblocks), "Not enough memory for check sums");
-    unsigned long long *part = (unsigned long long *)p;
-    CK_TRY(tmp.get(&p, (size_t)sgrid * (CBLOCK + 2)), "Not enough memory for check sums");
-    uint8_t *maps = (uint8_t *)p;
-    int32_t *perm = g->d_chk + 4 * (size_t)n;
-    int32_t *oob = g->d_chk + 5 * (size_t)n;
-    const int pgrid = (int)std::min<int64_t>((n + 255) / 256, (int64_t)g->num_cu * 8);
-    if (g->relabelled && !g->chk_perm) {
-        hipLaunchKernelGGL(k_c_perm, dim3(pgrid), dim3(256), 0, g->stream, (const int2 *)g->d_unperm, g->n, perm);
-        CK_TRY(hipGetLastError(), "k_c_perm");
-        g->chk_perm = true;
-    }
-    if (g->internal_stale) {
-        // The last plain run wrote only the caller-order result: bring the
-        // internal-order array the neighbour tests read up to date.
-        hipLaunchKernelGGL(k_c_permute, dim3(pgrid), dim3(256), 0, g->stream, (const int2 *)g->d_unperm, g->n,
-                           (const int32_t *)g->d_match_out, g->d_match);
-        CK_TRY(hipGetLastError(), "k_c_permute");
-        g->internal_stale = false;
-    }
-    const int32_t ng = voff && n_graphs > 0 ? n_graphs : 1;
-    CDev d;
-    std::memset(&d, 0, sizeof(d));
-    d.n = g->n;
-    d.ranges = g->d_ranges;
-    d.nbrs = g->d_nbrs;
-    d.adj = g->weighted ? g->d_adj : nullptr;
-    d.res = g->relabelled ? g->d_match_out : g->d_match;
-    d.mint = g->d_match;
-    d.perm = g->relabelled ? perm : nullptr;
-    d.count = g->d_chk;
-    d.first = g->d_chk + n;
-    d.second = g->d_chk + 2 * (size_t)n;
-    d.third = g->d_chk + 3 * (size_t)n;
-    d.oob = oob;
-    d.part = part;
-    d.tmap = maps;
-    d.bmap = maps + (size_t)sgrid * CBLOCK;
-    d.bentry = maps + (size_t)sgrid * (CBLOCK + 1);
-    CK_TRY(tmp.get(&p, sizeof(int32_t) * (size_t)ng), "Not enough memory for per-graph state");
-    d.gexit = (int32_t *)p;
-    if (voff && n_graphs > 0) {
-        d.voff = g->d_voff;
-        d.n_graphs = n_graphs;
-        CK_TRY(tmp.get(&p, sizeof(double) * (size_t)ng), "Not enough memory for per-graph sums");
-        d.gweight = (double *)p;
-        CK_TRY(tmp.get(&p, sizeof(unsigned long long) * (size_t)ng), "Not enough memory for per-graph sums");
-        d.gsize = (unsigned long long *)p;
-        CK_TRY(tmp.get(&p, sizeof(int32_t) * (size_t)ng), "Not enough memory for per-graph sums");
-        d.gbad = (int32_t *)p;
-    }
-    hipLaunchKernelGGL(k_c_zero, dim3(std::max(1, std::min(1024, (ng + 255) / 256))), dim3(256), 0, g->stream,
-                       d.gweight, d.gsize, d.gbad, d.gexit, ng);
-    CK_TRY(hipMemsetAsync(oob, 0, 2 * sizeof(int32_t), g->stream), "check counters");
-    hipLaunchKernelGGL(k_c_init, dim3(grid), dim3(CBLOCK), 0, g->stream, d);
-    CK_TRY(hipGetLastError(), "k_c_init");
-    int32_t h_oob[2] = {0, 0};
-    CK_TRY(hipMemcpyAsync(h_oob, oob, sizeof(h_oob), hipMemcpyDeviceToHost, g->stream), "check counters");
-    CK_TRY(hipStreamSynchronize(g->stream), "check counters");
-    if (h_oob[0] > 0) {
-        CK_TRY(tmp.get(&p, sizeof(int2) * (size_t)h_oob[0]), "Not enough memory for the check's value list");
-        d.oob_list = (int2 *)p;
-    }
-    for (int pass = 0; pass < 3; ++pass)
-        hipLaunchKernelGGL(k_c_group, dim3(grid), dim3(CBLOCK), 0, g->stream, d, pass);
-    hipLaunchKernelGGL(k_c_scan_maps, dim3(sgrid), dim3(CBLOCK), 0, g->stream, d);
-    CK_TRY(hipGetLastError(), "result check kernels");
-    // Chain the blocks' maps: the entry state of every block's first slot.
-    std::vector<uint8_t> bmap((size_t)sgrid), bentry((size_t)sgrid);
-    CK_TRY(hipMemcpyAsync(bmap.data(), d.bmap, bmap.size(), hipMemcpyDeviceToHost, g->stream), "check maps");
-    CK_TRY(hipStreamSynchronize(g->stream), "check maps");
-    int e = 0;
-    for (int b = 0; b < sgrid; ++b) {
-        bentry[(size_t)b] = (uint8_t)e;
-        e = (bmap[(size_t)b] >> e) & 1;
-    }
-    CK_TRY(hipMemcpyAsync(maps + (size_t)sgrid * (CBLOCK + 1), bentry.data(), bentry.size(), hipMemcpyHostToDevice,
-                          g->stream),
-           "check maps");
-    // k_c_scan_walk fills columns 0..2 of the first sgrid rows of the partial
-    // sums, k_c_vertices columns 3..5 of the first vgrid rows.
-    d.ipb = (int32_t)(((n + grid - 1) / grid + CBLOCK - 1) / CBLOCK * CBLOCK);
-    const int vgrid = (int)((n + d.ipb - 1) / d.ipb);
-    CK_TRY(hipMemsetAsync(part, 0, sizeof(unsigned long long) * NACC * (size_t)pblocks, g->stream), "check sums");
-    hipLaunchKernelGGL(k_c_scan_walk, dim3(sgrid), dim3(CBLOCK), 0, g->stream, d);
-    hipLaunchKernelGGL(k_c_vertices, dim3(vgrid), dim3(CBLOCK), 0, g->stream, d);
-    CK_TRY(hipGetLastError(), "result check kernels");
-    std::vector<unsigned long long> hpart((size_t)pblocks * NACC);
-    CK_TRY(hipMemcpyAsync(hpart.data(), part, sizeof(unsigned long long) * hpart.size(), hipMemcpyDeviceToHost,
-                          g->stream),
-           "Unable to read check sums");
-    std::vector<int32_t> gexit((size_t)ng);
-    CK_TRY(hipMemcpyAsync(gexit.data(), d.gexit, sizeof(int32_t) * gexit.size(), hipMemcpyDeviceToHost, g->stream),
-           "Unable to read check sums");
-    std::vector<double> gw;
-    std::vector<unsigned long long> gs;
-    std::vector<int32_t> gb((size_t)ng, 0);
-    if (d.voff) {
-        gw.resize((size_t)ng);
-        gs.resize((size_t)ng);
-        CK_TRY(hipMemcpyAsync(gw.data(), d.gweight, sizeof(double) * gw.size(), hipMemcpyDeviceToHost, g->stream),
-               "Unable to read check sums");
-        CK_TRY(hipMemcpyAsync(gs.data(), d.gsize, sizeof(unsigned long long) * gs.size(), hipMemcpyDeviceToHost,
-                              g->stream),
-               "Unable to read check sums");
-        CK_TRY(hipMemcpyAsync(gb.data(), d.gbad, sizeof(int32_t) * gb.size(), hipMemcpyDeviceToHost, g->stream),
-               "Unable to read check sums");
-    }
-    std::vector<int2> olist((size_t)h_oob[0]);
-    if (h_oob[0] > 0)
-        CK_TRY(hipMemcpyAsync(olist.data(), d.oob_list, sizeof(int2) * olist.size(), hipMemcpyDeviceToHost, g->stream),
-               "Unable to read the check's value list");
-    CK_TRY(hipStreamSynchronize(g->stream), "result check");
-    double weight = 0.0;
-    for (int b = 0; b < pblocks; ++b) {
-        const unsigned long long *q = hpart.data() + (size_t)b * NACC;
-        out->under += (int64_t)q[0];
-        out->multi += (int64_t)q[1];
-        out->invalid += (int64_t)q[2];
-        out->matchable += (int64_t)q[3];
-        out->size += (int64_t)q[4];
-        double w;
-        std::memcpy(&w, &q[5], sizeof(w));
-        weight += w;
-    }
-    if (!olist.empty()) {
-        // Values outside their graph's range: after every in-range value of the
-        // graph, in value order; walked from the state the scan left there.
-        std::vector<std::tuple<int32_t, int32_t, int32_t>> items;  // (graph, value, caller id)
-        items.reserve(olist.size());
-        for (const int2 &q : olist) {
-            int32_t gi = 0;
-            if (d.voff) gi = (int32_t)(std::upper_bound(voff, voff + n_graphs + 1, q.x) - voff) - 1;
-            items.emplace_back(gi, q.y, q.x);
-        }
-        std::sort(items.begin(), items.end());
-        std::vector<int2> pairs;
-        std::vector<int32_t> pair_graph;
-        size_t a = 0;
-        while (a < items.size()) {
-            const int32_t gi = std::get<0>(items[a]);
-            int ent = gexit[(size_t)gi];
-            while (a < items.size() && std::get<0>(items[a]) == gi) {
-                size_t b = a;
-                while (b < items.size() && std::get<0>(items[b]) == gi && std::get<1>(items[b]) == std::get<1>(items[a]))
-                    ++b;
-                const int32_t cnt = (int32_t)(b - a);
-                if (ent < cnt) {
-                    if (ent == cnt - 1) {
-                        ++out->under;
-                        gb[(size_t)gi] = 1;
-                    } else {
-                        pairs.push_back(make_int2(std::get<2>(items[a + ent]), std::get<2>(items[a + ent + 1])));
-                        pair_graph.push_back(gi);
-                        if (cnt - ent - 2 > 0) {
-                            out->multi += cnt - ent - 2;
-                            gb[(size_t)gi] = 1;
-                        }
-                    }
-                }
-                ent = next_entry(cnt, ent);
-                a = b;
-            }
-        }
-        if (!pairs.empty()) {
-            CK_TRY(tmp.get(&p, sizeof(int2) * pairs.size() + sizeof(int32_t) * pairs.size()),
-                   "Not enough memory for the check's pair list");
-            int2 *d_pairs = (int2 *)p;
-            int32_t *d_ok = (int32_t *)(d_pairs + pairs.size());
-            CK_TRY(hipMemcpyAsync(d_pairs, pairs.data(), sizeof(int2) * pairs.size(), hipMemcpyHostToDevice, g->stream),
-                   "check pairs");
-            const int cnt = (int)pairs.size();
-            hipLaunchKernelGGL(k_c_pairs, dim3(std::min(1024, (cnt + 255) / 256)), dim3(256), 0, g->stream, d,
-                               (const int2 *)d_pairs, cnt, d_ok);
-            CK_TRY(hipGetLastError(), "k_c_pairs");
-            std::vector<int32_t> ok(pairs.size());
-            CK_TRY(hipMemcpyAsync(ok.data(), d_ok, sizeof(int32_t) * ok.size(), hipMemcpyDeviceToHost, g->stream),
-                   "check pairs");
-            CK_TRY(hipStreamSynchronize(g->stream), "check pairs");
-            for (size_t k = 0; k < ok.size(); ++k)
-                if (!ok[k]) {
-                    ++out->invalid;
-                    gb[(size_t)pair_graph[k]] = 1;
-                }
-        }
-    }
-    out->weight = weight / 2.0;
-    out->valid = out->under == 0 && out->multi == 0 && out->invalid == 0;
-    for (int32_t k = 0; k < n_graphs && d.voff; ++k) {
-        if (gw_out) gw_out[k] = gw[(size_t)k] / 2.0;
-        if (gs_out) gs_out[k] = (int64_t)gs[(size_t)k];
-        if (gv_out) gv_out[k] = gb[(size_t)k] ? 0 : 1;
-    }
-    return MM_OK;
+    Scratch s;
+    s.base = (char *)g->d_chk;
+    s.perm_ready = &g->chk_perm;
+    s.g = g;
+    s.grow_keys = grow_handle_keys;
+    return run_check(g, voff, n_graphs, s, out, gw_out, gs_out, gv_out);
 }
 
 }  // namespace
@@ -692,7 +787,7 @@ int32_t mm_graph_check(mm_graph *g, mm_check *out)
 {
     mm::clear_error();
     if (!g) return mm::set_error(MM_ERR_INVALID, "mm_graph_check: graph is NULL");
-    return run_check(g, nullptr, 0, out, nullptr, nullptr, nullptr);
+    return check_with_handle_scratch(g, nullptr, 0, out, nullptr, nullptr, nullptr);
 }
 
 int32_t mm_graph_set_match(mm_graph *g, const int32_t *match)
@@ -728,7 +823,7 @@ int32_t mm_batch_check(mm_batch *b, mm_check *total, double *graph_weight, int64
     mm::clear_error();
     if (!b || !b->g) return mm::set_error(MM_ERR_INVALID, "mm_batch_check: batch is NULL");
     const int32_t ng = (int32_t)b->vertex_offsets.size() - 1;
-    return run_check(b->g, b->vertex_offsets.data(), ng, total, graph_weight, graph_size, graph_valid);
+    return check_with_handle_scratch(b->g, b->vertex_offsets.data(), ng, total, graph_weight, graph_size, graph_valid);
 }
 
 }  // extern "C"

@@ -64,10 +64,11 @@ struct mm_graph {
     int32_t unperm_off[9] = {0};     // group offsets in d_unperm
     int32_t *d_match_out = nullptr;  // result in the caller's numbering, when relabelled
     int32_t *d_vbase = nullptr;      // batches: first global id of each internal vertex's graph
-    int32_t *d_chk = nullptr;        // [4][n] scratch of the device result check: group counts, first and
-                                     // second members, and (last) caller id -> internal id
-    bool chk_perm = false;           // the last quarter of d_chk holds the caller -> internal map
-    unsigned long long *d_cpart = nullptr;  // per-block partial sums of the device result check
+    void *d_chk = nullptr;           // scratch of the device result check (mm_check.hip lays it out)
+    size_t chk_bytes = 0;
+    bool chk_perm = false;           // d_chk holds the caller -> internal id map
+    unsigned long long *d_chk_keys = nullptr;  // the check's sort keys of out-of-range values
+    size_t chk_key_cap = 0;
     bool relabelled = false;
     bool internal_stale = false;     // d_match is behind d_match_out (the plain rounds write only the latter)
     int32_t *d_voff = nullptr;
