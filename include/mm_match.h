@@ -211,6 +211,15 @@ typedef struct mm_check {
     int32_t valid;      /* under == multi == invalid == 0 */
 } mm_check;
 int32_t mm_graph_check(mm_graph *g, mm_check *out);
+/* mm_graph_check without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture
+ * aside -- the call synchronises its stream -- tight serving loops, harnesses
+ * that forbid hipMalloc).  Same results as mm_graph_check, for graphs of up to
+ * MM_CHECK_NOALLOC_MAX_N vertices (larger ones: MM_ERR_INVALID).  Its scratch
+ * is one 128 MiB device array per GPU, held until the process ends and shared
+ * by every call on that GPU: calls that overlap on one GPU race. */
+#define MM_CHECK_NOALLOC_MAX_N 4194304
+int32_t mm_graph_check_noalloc(mm_graph *g, mm_check *out);
 /* Replaces the handle's current result with match (host, n entries, caller's
  * order) -- to check an array computed elsewhere. */
 int32_t mm_graph_set_match(mm_graph *g, const int32_t *match);

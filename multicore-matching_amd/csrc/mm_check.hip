@@ -40,6 +40,7 @@
 #include <climits>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 
 #include "../../include/mm_match.h"
 #include "mm_graph_impl.h"
@@ -779,9 +780,65 @@ int32_t check_with_handle_scratch(mm_graph *g, const int32_t *voff, int32_t n_gr
     return run_check(g, voff, n_graphs, s, out, gw_out, gs_out, gv_out);
 }
 
+// Scratch of mm_graph_check_noalloc: a file-scope device array, one copy per
+// GPU the library runs on, held until the process ends.  Host code must not
+// pass `scratch` itself to a kernel (on the host it names a stand-in, not a
+// device address): it uses the address hipGetSymbolAddress returns for the
+// current device, fetched once per device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)128 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int32_t noalloc_scratch(int device, char **out)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return mm::set_error(MM_ERR_INVALID, "mm_graph_check_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        CK_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "mm_graph_check_noalloc: hipGetSymbolAddress");
+        if (!p) return mm::set_error(MM_ERR_HIP, "mm_graph_check_noalloc: no device address for the scratch array");
+        g_scratch_ptr[device] = p;
+    }
+    *out = (char *)g_scratch_ptr[device];
+    return MM_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int32_t mm_graph_check_noalloc(mm_graph *g, mm_check *out)
+{
+    mm::clear_error();
+    if (!g) return mm::set_error(MM_ERR_INVALID, "mm_graph_check_noalloc: graph is NULL");
+    if (!out) return mm::set_error(MM_ERR_INVALID, "mm_graph_check_noalloc: out is NULL");
+    std::memset(out, 0, sizeof(*out));
+    out->valid = 1;
+    if (g->n == 0) return MM_OK;
+    if (g->n > MM_CHECK_NOALLOC_MAX_N)
+        return mm::set_errorf(MM_ERR_INVALID, "mm_graph_check_noalloc: %d vertices exceed the limit of %d",
+                              (int)g->n, (int)MM_CHECK_NOALLOC_MAX_N);
+    CK_TRY(hipSetDevice(g->device), "hipSetDevice");
+    const size_t key_cap = pow2_at_least((size_t)g->n);
+    const Layout L = check_layout(g, 1, key_cap);
+    if (L.bytes > NOALLOC_SCRATCH_BYTES)
+        return mm::set_errorf(MM_ERR_INVALID, "mm_graph_check_noalloc: the check needs %zu bytes of scratch, %zu fit",
+                              L.bytes, NOALLOC_SCRATCH_BYTES);
+    char *base = nullptr;
+    const int32_t rc = noalloc_scratch(g->device, &base);
+    if (rc != MM_OK) return rc;
+    bool perm_ready = false;  // the array is shared by every handle on the device
+    Scratch s;
+    s.base = base;
+    s.key_cap = key_cap;
+    s.perm_ready = &perm_ready;
+    s.g = g;
+    return run_check(g, nullptr, 0, s, out, nullptr, nullptr, nullptr);
+}
 
 int32_t mm_graph_check(mm_graph *g, mm_check *out)
 {

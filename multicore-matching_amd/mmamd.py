@@ -31,6 +31,7 @@ MM_NR_MATCH_ROUNDS = 20
 MM_NR_MAX_MATCH_ROUNDS = 255
 MM_MAX_ROUNDS = 255
 MM_DEFAULT_BARRIER = 0x88B81733
+MM_CHECK_NOALLOC_MAX_N = 4194304
 MM_CREATE_KEEP_ORDER = 1
 MM_CREATE_PACKED_ROWS = 2
 MM_CREATE_NO_HEADS = 4
@@ -198,6 +199,7 @@ SYMBOLS = {
     "mm_host_match": (_I32, [_I32, _I32, _I64, _P, _P, _P, _P]),
     "mm_graph_check": (_I32, [_P, ctypes.POINTER(_Check)]),
     "mm_graph_set_match": (_I32, [_P, _P]),
+    "mm_graph_check_noalloc": (_I32, [_P, ctypes.POINTER(_Check)]),
     "mm_batch_check": (_I32, [_P, ctypes.POINTER(_Check), _P, _P, _P]),
     "mm_batch_set_match": (_I32, [_P, _P]),
     "mm_autotune": (_I32, [_P, ctypes.POINTER(_Opts), ctypes.POINTER(_TuneOpts), ctypes.POINTER(_TuneResult)]),
@@ -497,6 +499,13 @@ class Graph:
         """getWeight / testMatching of the current device result, on the device (mm_graph_check)."""
         c = _Check()
         _check(lib().mm_graph_check(self._h, ctypes.byref(c)))
+        return _check_dict(c)
+
+    def check_noalloc(self) -> dict:
+        """As check(), with the library's static scratch instead of allocated memory
+        (mm_graph_check_noalloc; graphs of up to MM_CHECK_NOALLOC_MAX_N vertices)."""
+        c = _Check()
+        _check(lib().mm_graph_check_noalloc(self._h, ctypes.byref(c)))
         return _check_dict(c)
 
     def set_match(self, match):
