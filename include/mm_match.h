@@ -163,6 +163,8 @@ int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_
                         const int64_t *edge_offsets, const int32_t *ranges,
                         const int32_t *neighbours, const float *weights, mm_batch **out);
 int32_t mm_batch_destroy(mm_batch *b);
+/* Properties of the batch's concatenated device graph (as mm_graph_get_info). */
+int32_t mm_batch_get_info(const mm_batch *b, mm_graph_info *info);
 /* rounds_out int32[n_graphs] (may be NULL): rounds graph g executed alone. */
 int32_t mm_batch_match(mm_batch *b, const mm_opts *opts, int32_t *match_out,
                        int32_t *rounds_out, mm_stats *stats);

@@ -1433,164 +1433,51 @@ void release(mm_graph *g)
     delete g;
 }
 
-// Host validation of the reference CSR (src/graph.h:41-60) and detection of
-// the properties that make PUSH exact.
-int32_t analyse_csr(mm_graph *g, const int32_t *ranges, const int32_t *nbrs, const float *w)
+// Host validation of the caller's row ranges (src/graph.h:41-60): every
+// range lies inside the entries (a batch: inside its graph's block).  Also the
+// largest degree and the degree histogram.  The neighbour ids are checked on
+// the device, before any kernel reads by them (mm_upload.hip).
+int32_t analyse_csr(mm_graph *g, const int32_t *ranges, const int32_t *voff, const int64_t *eoff, int32_t n_graphs)
 {
     const int32_t n = g->n;
     const int64_t nnz = g->nnz;
-    std::vector<int32_t> bad(64, 0), unsorted(64, 0), asym(64, 0), wasym(64, 0), maxdeg(64, 0);
-    std::vector<int64_t> fwd(64, 0), bwd(64, 0);
+    std::vector<int32_t> bad(64, 0), maxdeg(64, 0);
     std::vector<std::vector<int64_t>> hist(64, std::vector<int64_t>(33, 0));
-    constexpr int64_t GRAIN = 1024;  // rows per work item; power-law rows are skewed
-    mm::parallel_for_dynamic(n, GRAIN, [&](int64_t b, int64_t e, unsigned t) {
+    mm::parallel_for(n, [&](int64_t b, int64_t e, unsigned t) {
+        int32_t k = 0;
+        if (voff) k = (int32_t)(std::upper_bound(voff, voff + n_graphs + 1, (int32_t)b) - voff) - 1;
         for (int64_t i = b; i < e; ++i) {
+            int64_t rows_end = nnz;
+            if (voff) {
+                while (voff[k + 1] <= i) ++k;
+                rows_end = eoff[k + 1] - eoff[k];
+            }
             const int32_t lo = ranges[2 * i], hi = ranges[2 * i + 1];
-            if (lo < 0 || hi < lo || (int64_t)hi > nnz) { bad[t] = 1; continue; }
+            if (lo < 0 || hi < lo || (int64_t)hi > rows_end) { bad[t] = 1; continue; }
             const int32_t deg = hi - lo;
             maxdeg[t] = std::max(maxdeg[t], deg);
             if (deg > 0) ++hist[t][31 - __builtin_clz((unsigned)deg)];
-            for (int32_t j = lo; j < hi; ++j) {
-                if (nbrs[j] < 0 || nbrs[j] >= n) bad[t] = 1;
-                if (j > lo && nbrs[j] <= nbrs[j - 1]) unsorted[t] = 1;
-            }
         }
     });
     for (int t = 0; t < 64; ++t)
         if (bad[t]) return mm::set_error(MM_ERR_INVALID, "malformed CSR: range or neighbour id out of bounds");
-    bool sorted = true;
-    for (int t = 0; t < 64; ++t) sorted = sorted && !unsorted[t];
-    // Symmetry (and, for weights, w(u,v) == w(v,u)) with strictly ascending
-    // rows: every forward entry (i, u), i < u, must find i in u's row; the
-    // reverses found are distinct backward entries, so equal forward and
-    // backward counts make that a bijection.
-    if (sorted)
-        mm::parallel_for_dynamic(n, GRAIN, [&](int64_t b, int64_t e, unsigned t) {
-            for (int64_t i = b; i < e; ++i) {
-                const int32_t lo = ranges[2 * i], hi = ranges[2 * i + 1];
-                for (int32_t j = lo; j < hi; ++j) {
-                    const int32_t u = nbrs[j];
-                    if (u < (int32_t)i) {
-                        ++bwd[t];
-                        continue;
-                    }
-                    if (u == (int32_t)i) continue;
-                    ++fwd[t];
-                    const int32_t *rb = nbrs + ranges[2 * u], *re = nbrs + ranges[2 * u + 1];
-                    const int32_t *it = std::lower_bound(rb, re, (int32_t)i);
-                    if (it == re || *it != (int32_t)i) { asym[t] = 1; continue; }
-                    if (w) {
-                        const float a = w[j], c = w[it - nbrs];
-                        if (!(a == c || (a != a && c != c))) wasym[t] = 1;
-                    }
-                }
-            }
-        });
-    int64_t nf = 0, nbk = 0;
     for (int t = 0; t < 64; ++t) {
-        nf += fwd[t];
-        nbk += bwd[t];
-    }
-    if (nf != nbk) asym[0] = 1;
-    int us = 0, as = 0, ws = 0;
-    for (int t = 0; t < 64; ++t) {
-        us |= unsorted[t];
-        as |= asym[t];
-        ws |= wasym[t];
         g->max_degree = std::max(g->max_degree, maxdeg[t]);
         for (int k = 0; k < 33; ++k) g->deg_hist[k] += hist[t][k];
     }
-    g->push_ok_struct = !us && !as;
-    g->push_ok_weights = g->push_ok_struct && !ws;
     g->mean_degree = n > 0 ? (double)nnz / (double)n : 0.0;
     return MM_OK;
 }
 
-// Locality order of the vertices: compact regions of up to REGION vertices,
-// each grown breadth-first, the next one seeded beside the last (components
-// taken in increasing id order).  Neighbours get nearby positions, so the
-// per-round gathers of vertex state hit the same cache lines, and the buckets
-// (ranges of the order) keep most edges inside themselves: on a 2^22-vertex
-// RGG 97% of the edges join two ids of the same 8192-id range, against 76% for
-// one breadth-first sweep, whose long thin levels cross every range.
-// order[new] = old.  Components never mix, so every batched graph keeps one
-// contiguous block of new ids, in order.
-constexpr size_t REGION = 2048;
-std::vector<int32_t> locality_order(int32_t n, const int32_t *ranges, const int32_t *nbrs, const int32_t *vbase)
-{
-    // Regions over the vertices with neighbours; vertices without neighbours
-    // are then spread evenly over the order instead of trailing it.  Buckets
-    // are ranges of the order, and an isolated vertex costs a bucket almost
-    // nothing, so a graph with many of them (R-MAT: about half) would
-    // otherwise leave the giant component's buckets with all of the work.  A
-    // batch (vbase: first id of each vertex's graph) is ordered graph by
-    // graph, so every graph keeps a contiguous range of the order.
-    std::vector<int32_t> order((size_t)n), conn, iso, border;
-    std::vector<uint8_t> seen((size_t)n, 0);
-    int32_t lo = 0;
-    while (lo < n) {
-        int32_t hi = lo + 1;
-        if (vbase)
-            while (hi < n && vbase[(size_t)hi] != hi) ++hi;
-        else
-            hi = n;
-        conn.clear();
-        iso.clear();
-        for (int32_t s0 = lo; s0 < hi; ++s0) {
-            if (seen[(size_t)s0]) continue;
-            if (ranges[2 * (size_t)s0 + 1] == ranges[2 * (size_t)s0]) {
-                seen[(size_t)s0] = 1;
-                iso.push_back(s0);  // (a row naming it from elsewhere will not add it again)
-                continue;
-            }
-            border.assign(1, s0);
-            while (!border.empty()) {
-                const int32_t b = border.back();
-                border.pop_back();
-                if (seen[(size_t)b]) continue;
-                // one region, breadth-first from b until it holds REGION vertices
-                const size_t start = conn.size();
-                size_t head = start;
-                seen[(size_t)b] = 1;
-                conn.push_back(b);
-                while (head < conn.size() && conn.size() - start < REGION) {
-                    const int32_t v = conn[head++];
-                    for (int32_t j = ranges[2 * (size_t)v]; j < ranges[2 * (size_t)v + 1]; ++j) {
-                        const int32_t u = nbrs[j];
-                        if (seen[(size_t)u]) continue;
-                        seen[(size_t)u] = 1;
-                        conn.push_back(u);
-                        if (conn.size() - start == REGION) break;
-                    }
-                }
-                // the unseen neighbours of its last rows seed the next regions,
-                // the most recent first, so the next region grows beside it
-                for (size_t i = head > start ? head - 1 : start; i < conn.size(); ++i) {
-                    const int32_t v = conn[i];
-                    for (int32_t j = ranges[2 * (size_t)v]; j < ranges[2 * (size_t)v + 1]; ++j)
-                        if (!seen[(size_t)nbrs[j]]) border.push_back(nbrs[j]);
-                }
-            }
-        }
-        // [lo, hi) takes this segment's vertices: conn and iso hold hi - lo ids
-        // (a batch's rows stay inside their graph).
-        const int64_t len = hi - lo, k = (int64_t)iso.size();
-        size_t ci = 0, ii = 0;
-        for (int64_t p = 0; p < len; ++p) {
-            // position p holds an isolated vertex when floor((p + 1) k / len) steps up
-            const bool take_iso = ((p + 1) * k) / len > (p * k) / len;
-            order[(size_t)(lo + p)] = take_iso ? iso[ii++] : conn[ci++];
-        }
-        lo = hi;
-    }
-    return order;
-}
-
-// vbase_old: per ORIGINAL vertex, the first id of its batched graph (null for
-// a single graph).  relabel: renumber the vertices internally (locality_order).
+// A graph handle from the caller's CSR.  Batches (voff / eoff / n_graphs):
+// ranges and neighbour ids local to each graph.  relabel: renumber the
+// vertices internally in a locality order; align_rows: start a row at the
+// next 128-byte line when its first 8 entries would straddle one; heads: keep
+// the sorted rows' first 4 entries in a dense array.  The host checks the CSR;
+// the device builds everything else (mm_upload.hip).
 int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *nbrs,
-                     const float *w, const int32_t *vbase_old, int relabel, bool align_rows, bool heads,
-                     mm_graph **out)
+                     const float *w, const int32_t *voff, const int64_t *eoff, int32_t n_graphs, int relabel,
+                     bool align_rows, bool heads, mm_graph **out)
 {
     mm::clear_error();
     if (!out) return mm::set_error(MM_ERR_INVALID, "mm_graph_create: out is NULL");
@@ -1603,9 +1490,20 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     g->n = n;
     g->nnz = nnz;
     g->weighted = w != nullptr;
-    // The CSR is checked on the host before any device is touched.
-    int32_t rc = analyse_csr(g, ranges, nbrs, w);
+    mm_upload_mark(nullptr, nullptr);
+    int32_t rc = analyse_csr(g, ranges, voff, eoff, n_graphs);
     if (rc != MM_OK) { delete g; return rc; }
+    // 32-bit proposal keys hold (position << L) | local id, L = bits of the
+    // largest local id: possible only while the largest degree leaves room
+    int32_t max_loc = n > 0 ? n - 1 : 0;
+    if (voff && n_graphs > 0) {
+        max_loc = 0;
+        for (int32_t k = 0; k < n_graphs; ++k) max_loc = std::max(max_loc, voff[k + 1] - voff[k] - 1);
+    }
+    int locbits = 1;
+    while (locbits < 31 && ((int64_t)1 << locbits) <= (int64_t)max_loc) ++locbits;
+    const bool keys32_possible = w && locbits < 31 && (int64_t)g->max_degree < ((int64_t)1 << (32 - locbits));
+    mm_upload_mark("host checks", nullptr);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         delete g;
@@ -1624,8 +1522,14 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         return fail(mm::set_error(MM_ERR_HIP, "hipStreamCreate failed"));
     if (hipEventCreate(&g->ev_t1) != hipSuccess || hipEventCreate(&g->ev_t2) != hipSuccess)
         return fail(mm::set_error(MM_ERR_HIP, "hipEventCreate failed"));
+    mm_upload_mark("device setup", nullptr);
+    UploadState *up = nullptr;
+    if ((rc = mm_upload_begin(g, ranges, nbrs, w, voff, eoff, n_graphs, keys32_possible, &up)) != MM_OK) return fail(rc);
+    struct UpGuard {
+        UploadState *p;
+        ~UpGuard() { mm_upload_free(p); }
+    } up_guard{up};
     const size_t nn = (size_t)n;
-    if ((rc = dev_alloc(g, &g->d_ranges, nn, "neighbour ranges")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_match, nn, "match")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_req, nn, "requests")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_st, nn, "state")) != MM_OK) return fail(rc);
@@ -1649,201 +1553,8 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     if (hipHostMalloc((void **)&g->h_ctr, sizeof(int32_t) * ROUND_SLOTS * CTR_STRIDE) != hipSuccess ||
         hipHostMalloc((void **)&g->h_ctr64, sizeof(unsigned long long) * ROUND_SLOTS * CTR64_STRIDE) != hipSuccess)
         return fail(mm::set_error(MM_ERR_NOMEM, "pinned host allocation failed"));
-    if ((rc = dev_alloc(g, &g->d_loc, nn, "original ids")) != MM_OK) return fail(rc);
-    // Device copies of the CSR, rows in the internal vertex order, neighbour
-    // ids renumbered:
-    //   nbrs: each row in CSR order (random matching: first red neighbour);
-    //   adj (weighted graphs): {id, weight} pairs sorted by weight descending,
-    //     CSR position ascending, with the weights that never win (NaN and
-    //     <= -1, the reference scans with '>' from -1.0) last in CSR order --
-    //     so the heaviest candidate, first in CSR order on ties, is the first
-    //     candidate in the row.
-    g->relabelled = relabel != 0 && n > 1;
-    std::vector<int32_t> order;
-    if (g->relabelled) order = locality_order(n, ranges, nbrs, vbase_old);
-    std::vector<int32_t> inv;
-    if (g->relabelled) {
-        inv.resize(nn);
-        for (int32_t i = 0; i < n; ++i) inv[(size_t)order[(size_t)i]] = i;
-    }
-    auto old_of = [&](int64_t i) -> int32_t { return g->relabelled ? order[(size_t)i] : (int32_t)i; };
-    std::vector<int32_t> h_ranges(2 * nn), h_loc(nn);
-    std::vector<int32_t> h_vbase;
-    if (vbase_old) h_vbase.resize(nn);
-    // Row offsets of the device copies.  With align_rows a row whose first 8
-    // entries would straddle a 128-byte line of adj (16 entries) starts at the
-    // next line instead, so the first step of a scan over a lone row costs one
-    // line; the gap is never read.  Offsets must stay int32.
-    int64_t nnz_dev = 0;
-    for (int pass = align_rows ? 0 : 1; pass < 2; ++pass) {
-        int64_t off = 0;
-        for (int32_t i = 0; i < n; ++i) {
-            const int32_t o = old_of(i);
-            const int32_t deg = ranges[2 * (size_t)o + 1] - ranges[2 * (size_t)o];
-            if (pass == 0 && (off & 15) + std::min<int64_t>(deg, 8) > 16) off = (off + 15) & ~(int64_t)15;
-            h_ranges[2 * (size_t)i] = (int32_t)off;
-            h_ranges[2 * (size_t)i + 1] = (int32_t)(off + deg);
-            off += deg;
-        }
-        nnz_dev = off;
-        if (off <= INT32_MAX) break;
-    }
-    for (int32_t i = 0; i < n; ++i) {
-        const int32_t o = old_of(i);
-        if (vbase_old) h_vbase[(size_t)i] = vbase_old[(size_t)o];
-        h_loc[(size_t)i] = o - (vbase_old ? vbase_old[(size_t)o] : 0);
-    }
-    if ((rc = dev_alloc(g, &g->d_nbrs, (size_t)nnz_dev, "neighbours")) != MM_OK) return fail(rc);
-    if (g->weighted && (rc = dev_alloc(g, &g->d_adj, (size_t)nnz_dev, "sorted weighted rows")) != MM_OK)
-        return fail(rc);
-    std::vector<int32_t> h_nbrs((size_t)nnz_dev, 0);
-    std::vector<int2> h_adj;
-    if (g->weighted) h_adj.assign((size_t)nnz_dev, make_int2(0, 0));
-    // 32-bit proposal keys need the position of every edge in the other end's
-    // sorted row (PUSH graphs: symmetric, ascending rows, symmetric weights)
-    // and room for (position << L) | local id with L bits of local id.
-    int32_t max_loc = 0;
-    for (int32_t i = 0; i < n; ++i) max_loc = std::max(max_loc, h_loc[(size_t)i]);
-    int locbits = 1;
-    while (locbits < 31 && ((int64_t)1 << locbits) <= (int64_t)max_loc) ++locbits;
-    // (an edgeless graph keeps no position rows: the device then uses 64-bit keys,
-    // and its heads must hold weights like its rows)
-    const bool key32 = g->weighted && g->push_ok_weights && n > 0 && nnz_dev > 0 && locbits < 31 &&
-                       (int64_t)g->max_degree < ((int64_t)1 << (32 - locbits));
-    std::vector<int32_t> h_spos;  // sorted position of each row entry in CSR order
-    if (key32) h_spos.assign((size_t)nnz_dev, 0);
-    mm::parallel_for_dynamic(n, 1024, [&](int64_t b, int64_t e, unsigned) {
-        std::vector<int32_t> perm;
-        for (int64_t i = b; i < e; ++i) {
-            const int32_t o = old_of(i);
-            const int32_t src = ranges[2 * (size_t)o];
-            const int32_t deg = ranges[2 * (size_t)o + 1] - src;
-            const int32_t dst = h_ranges[2 * (size_t)i];
-            for (int32_t k = 0; k < deg; ++k) {
-                const int32_t u = nbrs[src + k];
-                h_nbrs[(size_t)dst + k] = g->relabelled ? inv[(size_t)u] : u;
-            }
-            if (!g->weighted) continue;
-            perm.resize((size_t)deg);
-            for (int32_t k = 0; k < deg; ++k) perm[(size_t)k] = k;
-            const float *wr = w + src;
-            std::stable_sort(perm.begin(), perm.end(), [&](int32_t x, int32_t y) {
-                const bool vx = wr[x] > -1.0f, vy = wr[y] > -1.0f;
-                if (vx != vy) return vx;
-                return vx && wr[x] > wr[y];
-            });
-            for (int32_t k = 0; k < deg; ++k) {
-                const float wk = wr[perm[(size_t)k]];
-                const int32_t id = h_nbrs[(size_t)dst + perm[(size_t)k]];
-                int32_t wb;
-                std::memcpy(&wb, &wk, sizeof(wb));
-                h_adj[(size_t)dst + k] = make_int2(wk > -1.0f ? id : ~id, wb);
-                if (key32) h_spos[(size_t)dst + perm[(size_t)k]] = k;
-            }
-        }
-    });
-    // Rows of the plain scans with 32-bit keys: {id, position of this vertex
-    // in the neighbour's sorted row}.  Row u's entry at sorted position p names
-    // v; it gives the position field of v's entry for u, found by a binary
-    // search of the ascending CSR row of v.
-    std::vector<int2> h_adjp;
-    if (key32) {
-        h_adjp = h_adj;
-        mm::parallel_for_dynamic(n, 1024, [&](int64_t b, int64_t e, unsigned) {
-            for (int64_t u = b; u < e; ++u) {
-                const int32_t ou = old_of(u);
-                const int32_t dst = h_ranges[2 * (size_t)u], deg = h_ranges[2 * (size_t)u + 1] - dst;
-                for (int32_t p = 0; p < deg; ++p) {
-                    const int32_t x = h_adj[(size_t)dst + p].x, v = x >= 0 ? x : ~x;
-                    const int32_t ov = old_of(v);
-                    const int32_t *rb = nbrs + ranges[2 * (size_t)ov], *re = nbrs + ranges[2 * (size_t)ov + 1];
-                    const int32_t c = (int32_t)(std::lower_bound(rb, re, ou) - rb);
-                    const size_t at = (size_t)h_ranges[2 * (size_t)v] + (size_t)h_spos[(size_t)h_ranges[2 * (size_t)v] + c];
-                    h_adjp[at].y = p;
-                }
-            }
-        });
-        g->locbits = locbits;
-    }
-    const std::vector<int2> &h_scan = key32 ? h_adjp : h_adj;  // rows the plain scans read
-    if (n > 0 && (hipMemcpy(g->d_ranges, h_ranges.data(), sizeof(int2) * nn, hipMemcpyHostToDevice) != hipSuccess ||
-                  hipMemcpy(g->d_loc, h_loc.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess))
-        return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
-    if (g->weighted && heads && n > 0) {
-        // Dense copy of each sorted row's first 4 entries (the gap of a shorter
-        // row is never read: scans stop at the row end).
-        std::vector<int2> h_head(4 * nn, make_int2(0, 0));
-        mm::parallel_for_dynamic(n, 4096, [&](int64_t b, int64_t e, unsigned) {
-            for (int64_t i = b; i < e; ++i) {
-                const int32_t lo = h_ranges[2 * (size_t)i], hi = h_ranges[2 * (size_t)i + 1];
-                for (int32_t k = 0; k < 4 && lo + k < hi; ++k) h_head[4 * (size_t)i + k] = h_scan[(size_t)lo + k];
-            }
-        });
-        // 4-byte heads when the entries carry key positions and id + position fit
-        // in 31 bits (the scans read only those then), else 8-byte heads
-        int pb = 1, ib = 1;
-        while (pb < 31 && ((int64_t)1 << pb) < (int64_t)g->max_degree) ++pb;
-        while (ib < 31 && ((int64_t)1 << ib) < (int64_t)n) ++ib;
-        if (key32 && ib + pb <= 31) {
-            std::vector<uint32_t> h_headc(4 * nn, 0u);
-            mm::parallel_for_dynamic(n, 4096, [&](int64_t b, int64_t e, unsigned) {
-                for (int64_t i = b; i < e; ++i)
-                    for (int k = 0; k < 4; ++k) {
-                        const int2 x = h_head[4 * (size_t)i + k];
-                        const uint32_t id = (uint32_t)(x.x >= 0 ? x.x : ~x.x);
-                        h_headc[4 * (size_t)i + k] =
-                            (x.x < 0 ? 0x80000000u : 0u) | (id << pb) | ((uint32_t)x.y & ((1u << pb) - 1u));
-                    }
-            });
-            if ((rc = dev_alloc(g, &g->d_headc, 4 * nn, "row heads")) != MM_OK) return fail(rc);
-            if (hipMemcpy(g->d_headc, h_headc.data(), sizeof(uint32_t) * 4 * nn, hipMemcpyHostToDevice) != hipSuccess)
-                return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
-            g->head_pb = pb;
-        } else {
-            if ((rc = dev_alloc(g, &g->d_head, 4 * nn, "row heads")) != MM_OK) return fail(rc);
-            if (hipMemcpy(g->d_head, h_head.data(), sizeof(int2) * 4 * nn, hipMemcpyHostToDevice) != hipSuccess)
-                return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
-        }
-    }
-    if (nnz_dev > 0) {
-        if (hipMemcpy(g->d_nbrs, h_nbrs.data(), sizeof(int32_t) * (size_t)nnz_dev, hipMemcpyHostToDevice) !=
-                hipSuccess ||
-            (g->weighted &&
-             hipMemcpy(g->d_adj, h_adj.data(), sizeof(int2) * (size_t)nnz_dev, hipMemcpyHostToDevice) != hipSuccess))
-            return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
-        if (key32) {
-            if ((rc = dev_alloc(g, &g->d_adjp, (size_t)nnz_dev, "sorted rows with key positions")) != MM_OK)
-                return fail(rc);
-            if (hipMemcpy(g->d_adjp, h_adjp.data(), sizeof(int2) * (size_t)nnz_dev, hipMemcpyHostToDevice) !=
-                hipSuccess)
-                return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
-        }
-    }
-    if (vbase_old && n > 0) {
-        if ((rc = dev_alloc(g, &g->d_vbase, nn, "graph bases")) != MM_OK) return fail(rc);
-        if (hipMemcpy(g->d_vbase, h_vbase.data(), sizeof(int32_t) * nn, hipMemcpyHostToDevice) != hipSuccess)
-            return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
-    }
-    if (g->relabelled) {
-        // (output position, internal id) pairs grouped by the eighth of the
-        // internal ids, ordered by output position inside a group.
-        std::vector<int2> pairs(nn);
-        int32_t cnt[8] = {0};
-        auto eighth = [&](int32_t v) { return (int)(((int64_t)v * 8) / n); };
-        for (int32_t o = 0; o < n; ++o) ++cnt[eighth(inv[(size_t)o])];
-        g->unperm_off[0] = 0;
-        for (int x = 0; x < 8; ++x) g->unperm_off[x + 1] = g->unperm_off[x] + cnt[x];
-        int32_t pos[8];
-        for (int x = 0; x < 8; ++x) pos[x] = g->unperm_off[x];
-        for (int32_t o = 0; o < n; ++o) {
-            const int32_t v = inv[(size_t)o];
-            pairs[(size_t)pos[eighth(v)]++] = make_int2(o, v);
-        }
-        if ((rc = dev_alloc(g, &g->d_unperm, nn, "vertex order")) != MM_OK) return fail(rc);
-        if ((rc = dev_alloc(g, &g->d_match_out, nn, "match")) != MM_OK) return fail(rc);
-        if (hipMemcpy(g->d_unperm, pairs.data(), sizeof(int2) * nn, hipMemcpyHostToDevice) != hipSuccess)
-            return fail(mm::set_error(MM_ERR_HIP, "Unable to transfer graph data to device"));
-    }
+    mm_upload_mark("matching buffers", nullptr);
+    if ((rc = mm_upload_finish(g, up, relabel, align_rows, heads)) != MM_OK) return fail(rc);
     *out = g;
     return MM_OK;
 }
@@ -2446,7 +2157,7 @@ int32_t mm_device_count(void)
 int32_t mm_graph_create(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *neighbours,
                         const float *weights, mm_graph **out)
 {
-    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, 1, true, true, out);
+    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, nullptr, 0, 1, true, true, out);
 }
 
 int32_t mm_graph_create_ex(int32_t device, int32_t n, int64_t nnz, const int32_t *ranges, const int32_t *neighbours,
@@ -2454,7 +2165,7 @@ int32_t mm_graph_create_ex(int32_t device, int32_t n, int64_t nnz, const int32_t
 {
     if (flags & ~(uint32_t)(MM_CREATE_KEEP_ORDER | MM_CREATE_PACKED_ROWS | MM_CREATE_NO_HEADS))
         return mm::set_error(MM_ERR_INVALID, "mm_graph_create_ex: bad flags");
-    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, (flags & MM_CREATE_KEEP_ORDER) ? 0 : 1,
+    return create_graph(device, n, nnz, ranges, neighbours, weights, nullptr, nullptr, 0, (flags & MM_CREATE_KEEP_ORDER) ? 0 : 1,
                         (flags & MM_CREATE_PACKED_ROWS) == 0, (flags & MM_CREATE_NO_HEADS) == 0, out);
 }
 
@@ -2514,34 +2225,16 @@ int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_
     for (int32_t k = 0; k < n_graphs; ++k)
         if (vertex_offsets[k + 1] < vertex_offsets[k] || edge_offsets[k + 1] < edge_offsets[k])
             return mm::set_error(MM_ERR_INVALID, "mm_batch_create: offsets must be non-decreasing");
-    // Rebase every graph into one global CSR.
-    std::vector<int32_t> granges((size_t)total_n * 2), gnbrs((size_t)total_nnz);
-    std::vector<int32_t> vbase((size_t)total_n);
-    int bad = 0;
-    mm::parallel_for(n_graphs, [&](int64_t b, int64_t e, unsigned) {
-        for (int64_t k = b; k < e; ++k) {
-            const int32_t v0 = vertex_offsets[k], v1 = vertex_offsets[k + 1];
-            const int64_t e0 = edge_offsets[k], e1 = edge_offsets[k + 1];
-            const int32_t nk = v1 - v0;
-            for (int32_t v = v0; v < v1; ++v) {
-                const int64_t lo = ranges[2 * (size_t)v], hi = ranges[2 * (size_t)v + 1];
-                if (lo < 0 || hi < lo || e0 + hi > e1) { bad = 1; continue; }
-                granges[2 * (size_t)v] = (int32_t)(e0 + lo);
-                granges[2 * (size_t)v + 1] = (int32_t)(e0 + hi);
-                vbase[(size_t)v] = v0;
-            }
-            for (int64_t j = e0; j < e1; ++j) {
-                const int32_t u = neighbours[j];
-                if (u < 0 || u >= nk) { bad = 1; continue; }
-                gnbrs[(size_t)j] = u + v0;
-            }
-        }
-    });
-    if (bad) return mm::set_error(MM_ERR_INVALID, "mm_batch_create: malformed per-graph CSR");
+    // The graphs' rows and ids stay local to each graph; the device rebases
+    // them into one CSR (mm_upload.hip).
     mm_graph *g = nullptr;
-    int32_t rc = create_graph(device, (int32_t)total_n, total_nnz, granges.data(), gnbrs.data(), weights,
-                              vbase.data(), 1, true, true, &g);
-    if (rc != MM_OK) return rc;
+    int32_t rc = create_graph(device, (int32_t)total_n, total_nnz, ranges, neighbours, weights, vertex_offsets,
+                              edge_offsets, n_graphs, 1, true, true, &g);
+    if (rc != MM_OK) {
+        if (rc == MM_ERR_INVALID && std::strstr(mm_last_error(), "malformed CSR"))
+            return mm::set_error(MM_ERR_INVALID, "mm_batch_create: malformed per-graph CSR");
+        return rc;
+    }
     g->n_graphs = n_graphs;
     if ((rc = dev_alloc(g, &g->d_voff, (size_t)n_graphs + 1, "batch offsets")) != MM_OK ||
         (rc = dev_alloc(g, &g->d_vround, (size_t)total_n, "batch rounds")) != MM_OK ||
@@ -2563,6 +2256,12 @@ int32_t mm_batch_create(int32_t device, int32_t n_graphs, const int32_t *vertex_
     b->vertex_offsets.assign(vertex_offsets, vertex_offsets + n_graphs + 1);
     *out = b;
     return MM_OK;
+}
+
+int32_t mm_batch_get_info(const mm_batch *b, mm_graph_info *info)
+{
+    if (!b || !b->g) return mm::set_error(MM_ERR_INVALID, "mm_batch_get_info: NULL argument");
+    return mm_graph_get_info(b->g, info);
 }
 
 int32_t mm_batch_destroy(mm_batch *b)

@@ -189,6 +189,7 @@ SYMBOLS = {
     "mm_graph_copy_match": (_I32, [_P, _P]),
     "mm_batch_create": (_I32, [_I32, _I32, _P, _P, _P, _P, _P, ctypes.POINTER(_P)]),
     "mm_batch_destroy": (_I32, [_P]),
+    "mm_batch_get_info": (_I32, [_P, ctypes.POINTER(_Info)]),
     "mm_batch_match": (_I32, [_P, ctypes.POINTER(_Opts), _P, _P, ctypes.POINTER(_Stats)]),
     "mm_gen_rgg": (_I32, [_I32, _D, _U64, _I32, ctypes.POINTER(_Csr)]),
     "mm_gen_rmat": (_I32, [_I32, _I32, _D, _D, _D, _U64, _I32, ctypes.POINTER(_Csr)]),
@@ -592,6 +593,11 @@ class Batch:
                                        ctypes.byref(r)))
         del keep
         return _tune_dict(r)
+
+    def info(self) -> dict:
+        i = _Info()
+        _check(lib().mm_batch_get_info(self._h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in _Info._fields_}
 
     def _set_match(self, match):
         """Replaces the device result with the graphs' arrays concatenated (mm_batch_set_match)."""

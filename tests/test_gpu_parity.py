@@ -261,6 +261,23 @@ def test_batch_equals_single_graph_path():
         assert res.rounds == max(rounds)
 
 
+def test_malformed_ids_are_rejected_on_the_device():
+    """Neighbour ids outside the graph (or a batch graph's local range) are
+    rejected before any kernel reads by them; the handle is not created."""
+    for bad in (CSR.from_offsets([0, 1, 2], np.array([1, 5], np.int32)),
+                CSR.from_offsets([0, 1, 2], np.array([1, -1], np.int32)),
+                # a row of more than 256 entries (checked by a whole block) holding id 7
+                CSR.from_offsets([0, 300, 301], np.concatenate([np.ones(299), [7], [0]]).astype(np.int32))):
+        with pytest.raises(mmamd.MatchError) as e:
+            Graph(bad)
+        assert e.value.code == mmamd.MM_ERR_INVALID and "out of bounds" in str(e.value)
+    good = CSR.from_offsets([0, 1, 2], np.array([1, 0], np.int32))
+    with pytest.raises(mmamd.MatchError) as e:  # graph 1's id 1 is local: past its single vertex
+        Batch([good, CSR.from_offsets([0, 1], np.array([1], np.int32))])
+    assert e.value.code == mmamd.MM_ERR_INVALID
+    assert Graph(good).match(seeds=SEEDS).match.tolist() == [4, 4]
+
+
 def test_empty_graph_and_bad_arguments():
     say("edge cases")
     g = Graph(CSR.from_offsets([0], np.zeros(0, np.int32)))
