@@ -1433,42 +1433,6 @@ void release(mm_graph *g)
     delete g;
 }
 
-// Host validation of the caller's row ranges (src/graph.h:41-60): every
-// range lies inside the entries (a batch: inside its graph's block).  Also the
-// largest degree and the degree histogram.  The neighbour ids are checked on
-// the device, before any kernel reads by them (mm_upload.hip).
-int32_t analyse_csr(mm_graph *g, const int32_t *ranges, const int32_t *voff, const int64_t *eoff, int32_t n_graphs)
-{
-    const int32_t n = g->n;
-    const int64_t nnz = g->nnz;
-    std::vector<int32_t> bad(64, 0), maxdeg(64, 0);
-    std::vector<std::vector<int64_t>> hist(64, std::vector<int64_t>(33, 0));
-    mm::parallel_for(n, [&](int64_t b, int64_t e, unsigned t) {
-        int32_t k = 0;
-        if (voff) k = (int32_t)(std::upper_bound(voff, voff + n_graphs + 1, (int32_t)b) - voff) - 1;
-        for (int64_t i = b; i < e; ++i) {
-            int64_t rows_end = nnz;
-            if (voff) {
-                while (voff[k + 1] <= i) ++k;
-                rows_end = eoff[k + 1] - eoff[k];
-            }
-            const int32_t lo = ranges[2 * i], hi = ranges[2 * i + 1];
-            if (lo < 0 || hi < lo || (int64_t)hi > rows_end) { bad[t] = 1; continue; }
-            const int32_t deg = hi - lo;
-            maxdeg[t] = std::max(maxdeg[t], deg);
-            if (deg > 0) ++hist[t][31 - __builtin_clz((unsigned)deg)];
-        }
-    });
-    for (int t = 0; t < 64; ++t)
-        if (bad[t]) return mm::set_error(MM_ERR_INVALID, "malformed CSR: range or neighbour id out of bounds");
-    for (int t = 0; t < 64; ++t) {
-        g->max_degree = std::max(g->max_degree, maxdeg[t]);
-        for (int k = 0; k < 33; ++k) g->deg_hist[k] += hist[t][k];
-    }
-    g->mean_degree = n > 0 ? (double)nnz / (double)n : 0.0;
-    return MM_OK;
-}
-
 // A graph handle from the caller's CSR.  Batches (voff / eoff / n_graphs):
 // ranges and neighbour ids local to each graph.  relabel: renumber the
 // vertices internally in a locality order; align_rows: start a row at the
@@ -1491,19 +1455,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     g->nnz = nnz;
     g->weighted = w != nullptr;
     mm_upload_mark(nullptr, nullptr);
-    int32_t rc = analyse_csr(g, ranges, voff, eoff, n_graphs);
-    if (rc != MM_OK) { delete g; return rc; }
-    // 32-bit proposal keys hold (position << L) | local id, L = bits of the
-    // largest local id: possible only while the largest degree leaves room
-    int32_t max_loc = n > 0 ? n - 1 : 0;
-    if (voff && n_graphs > 0) {
-        max_loc = 0;
-        for (int32_t k = 0; k < n_graphs; ++k) max_loc = std::max(max_loc, voff[k + 1] - voff[k] - 1);
-    }
-    int locbits = 1;
-    while (locbits < 31 && ((int64_t)1 << locbits) <= (int64_t)max_loc) ++locbits;
-    const bool keys32_possible = w && locbits < 31 && (int64_t)g->max_degree < ((int64_t)1 << (32 - locbits));
-    mm_upload_mark("host checks", nullptr);
+    int32_t rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         delete g;
@@ -1524,7 +1476,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         return fail(mm::set_error(MM_ERR_HIP, "hipEventCreate failed"));
     mm_upload_mark("device setup", nullptr);
     UploadState *up = nullptr;
-    if ((rc = mm_upload_begin(g, ranges, nbrs, w, voff, eoff, n_graphs, keys32_possible, &up)) != MM_OK) return fail(rc);
+    if ((rc = mm_upload_begin(g, ranges, nbrs, w, voff, eoff, n_graphs, relabel, &up)) != MM_OK) return fail(rc);
     struct UpGuard {
         UploadState *p;
         ~UpGuard() { mm_upload_free(p); }
@@ -1554,7 +1506,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         hipHostMalloc((void **)&g->h_ctr64, sizeof(unsigned long long) * ROUND_SLOTS * CTR64_STRIDE) != hipSuccess)
         return fail(mm::set_error(MM_ERR_NOMEM, "pinned host allocation failed"));
     mm_upload_mark("matching buffers", nullptr);
-    if ((rc = mm_upload_finish(g, up, relabel, align_rows, heads)) != MM_OK) return fail(rc);
+    if ((rc = mm_upload_finish(g, up, align_rows, heads)) != MM_OK) return fail(rc);
     *out = g;
     return MM_OK;
 }

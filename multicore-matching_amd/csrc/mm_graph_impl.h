@@ -103,20 +103,22 @@ struct mm_batch {
 int32_t mm_graph_alloc(mm_graph *g, void **p, size_t bytes, const char *what);
 
 // Device-side construction of a handle's arrays (mm_upload.hip).  begin copies
-// the caller's CSR to the device, checks its neighbour ids (MM_ERR_INVALID
-// when one lies outside its graph), rebases a batch's per-graph rows into one
-// CSR by voff / eoff and, when every row is strictly ascending, tests the
-// symmetry that proposal atomics need (sets push_ok_*; want_rev: also keeps
-// every entry's reverse for the 32-bit keys); finish builds the internal
-// order, rows, sorted rows, key positions, heads and caller-order pairs.
-// mm_upload_free releases the temporaries.
+// the caller's CSR to the device, checks its row ranges and neighbour ids
+// (MM_ERR_INVALID when one leaves its graph) and records the degree
+// statistics, rebases a batch's per-graph rows into one CSR by voff / eoff,
+// computes the internal order (relabel: the locality order, else the
+// caller's) and, when every row is strictly ascending, tests the symmetry that
+// proposal atomics need (sets push_ok_*; also keeps every entry's reverse
+// when the graph can take 32-bit keys); finish builds the rows, sorted rows,
+// key positions, heads and caller-order pairs.  mm_upload_free releases the
+// temporaries.
 struct UploadState;
 // Upload phase times on stderr when the environment sets MM_UPLOAD_TIMING:
 // the time since the previous mark (stream s drained first; null: none).
 void mm_upload_mark(const char *phase, hipStream_t s);
 int32_t mm_upload_begin(mm_graph *g, const int32_t *ranges, const int32_t *nbrs, const float *w, const int32_t *voff,
-                        const int64_t *eoff, int32_t n_graphs, bool want_rev, UploadState **out);
-int32_t mm_upload_finish(mm_graph *g, UploadState *u, int relabel, bool align_rows, bool heads);
+                        const int64_t *eoff, int32_t n_graphs, int relabel, UploadState **out);
+int32_t mm_upload_finish(mm_graph *g, UploadState *u, bool align_rows, bool heads);
 void mm_upload_free(UploadState *u);
 // Enqueues the copy of the internal match array into the caller's vertex order
 // (relabelled graphs only; a no-op otherwise).

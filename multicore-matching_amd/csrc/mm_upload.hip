@@ -34,6 +34,7 @@
 #include <cstdint>
 #include <cstring>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "../../include/mm_match.h"
@@ -117,7 +118,7 @@ constexpr int32_t LONG_ROW = 256;
 
 // Per entry of row i (graph of ids [0, ids_end)): the id is inside the graph
 // and larger than the previous entry's.
-__device__ __forceinline__ void check_entry(const int32_t *nb, int32_t lo, int32_t j, int32_t ids_end, int &bad,
+__device__ __forceinline__ void check_entry(const int32_t *nb, int64_t lo, int64_t j, int32_t ids_end, int &bad,
                                             int &unsorted)
 {
     const int32_t u = nb[j];
@@ -125,47 +126,80 @@ __device__ __forceinline__ void check_entry(const int32_t *nb, int32_t lo, int32
     if (j > lo && u <= nb[j - 1]) unsorted = 1;
 }
 
-__device__ __forceinline__ int32_t ids_end_of(const int32_t *voff, int32_t ng, int32_t n, int64_t i)
+// Row i's entries [lo, hi) in the copied arrays and its graph's id count, or
+// false when its range leaves the entries (a batch: its graph's block; the
+// ranges are still local to the graph then).
+__device__ __forceinline__ bool row_span(const int2 *rg, int64_t i, int32_t n, int64_t nnz, const int32_t *voff,
+                                         const long long *eoff, int32_t ng, int64_t &lo, int64_t &hi, int32_t &ids_end)
 {
-    if (!voff) return n;
-    const int32_t k = graph_of(voff, ng, (int32_t)i);
-    return voff[k + 1] - voff[k];
+    const int2 r = rg[i];
+    int64_t e0 = 0, e1 = nnz;
+    ids_end = n;
+    if (voff) {
+        const int32_t k = graph_of(voff, ng, (int32_t)i);
+        e0 = eoff[k];
+        e1 = eoff[k + 1];
+        ids_end = voff[k + 1] - voff[k];
+    }
+    lo = e0 + r.x;
+    hi = e0 + r.y;
+    return r.x >= 0 && r.y >= r.x && hi <= e1;
 }
 
-// The device half of the CSR validation (the host checked the row ranges):
-// neighbour ids (local to their graph in a batch) and ascending rows; rows
-// longer than LONG_ROW are listed for k_u_check_long.
-__global__ void k_u_check(const int2 *rg, const int32_t *nb, int32_t n, const int32_t *voff, int32_t ng, int32_t *flags,
-                          int32_t *longs, int32_t *nlong)
+// The CSR validation, before any kernel reads by the caller's values: row
+// ranges inside the entries, neighbour ids inside the graph (local to their
+// graph in a batch), and whether every row ascends.  Also the largest degree
+// (flags[3]) and the degree histogram; rows longer than LONG_ROW are listed
+// for k_u_check_long.
+__global__ __launch_bounds__(UB) void k_u_check(const int2 *rg, const int32_t *nb, int32_t n, int64_t nnz,
+                                                const int32_t *voff, const long long *eoff, int32_t ng, int32_t *flags,
+                                                int32_t *longs, unsigned long long *hist)
 {
+    __shared__ unsigned long long s_hist[33];
+    if (threadIdx.x < 33) s_hist[threadIdx.x] = 0;
+    __syncthreads();
     const int gl = threadIdx.x & 7;
     const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 3);
     int bad = 0, unsorted = 0;
+    int32_t maxdeg = 0;
     for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3); i < n; i += groups) {
-        const int2 r = rg[i];
-        if (r.y - r.x > LONG_ROW) {
-            if (gl == 0) longs[atomicAdd(nlong, 1)] = (int32_t)i;
+        int64_t lo, hi;
+        int32_t ids_end;
+        if (!row_span(rg, i, n, nnz, voff, eoff, ng, lo, hi, ids_end)) {
+            bad = 1;
             continue;
         }
-        const int32_t ids_end = ids_end_of(voff, ng, n, i);
-        for (int32_t j = r.x + gl; j < r.y; j += 8) check_entry(nb, r.x, j, ids_end, bad, unsorted);
+        const int32_t deg = (int32_t)(hi - lo);
+        if (gl == 0) {
+            maxdeg = max(maxdeg, deg);
+            if (deg > 0) atomicAdd(s_hist + (31 - __clz((unsigned)deg)), 1ull);
+        }
+        if (deg > LONG_ROW) {
+            if (gl == 0) longs[atomicAdd(flags + 2, 1)] = (int32_t)i;
+            continue;
+        }
+        for (int64_t j = lo + gl; j < hi; j += 8) check_entry(nb, lo, j, ids_end, bad, unsorted);
     }
+    for (int o = 32; o > 0; o >>= 1) maxdeg = max(maxdeg, __shfl_xor(maxdeg, o));
     const unsigned long long a = __ballot(bad), b = __ballot(unsorted);
     if ((threadIdx.x & 63) == 0) {
         if (a) atomicOr(flags, 1);
         if (b) atomicOr(flags + 1, 1);
+        if (maxdeg) atomicMax(flags + 3, maxdeg);
     }
+    __syncthreads();
+    if (threadIdx.x < 33 && s_hist[threadIdx.x]) atomicAdd(hist + threadIdx.x, s_hist[threadIdx.x]);
 }
 
-__global__ void k_u_check_long(const int2 *rg, const int32_t *nb, int32_t n, const int32_t *voff, int32_t ng,
-                               int32_t *flags, const int32_t *longs, int32_t nlong)
+__global__ void k_u_check_long(const int2 *rg, const int32_t *nb, int32_t n, int64_t nnz, const int32_t *voff,
+                               const long long *eoff, int32_t ng, int32_t *flags, const int32_t *longs, int32_t nlong)
 {
     int bad = 0, unsorted = 0;
     for (int32_t k = blockIdx.x; k < nlong; k += gridDim.x) {
-        const int32_t i = longs[k];
-        const int2 r = rg[i];
-        const int32_t ids_end = ids_end_of(voff, ng, n, i);
-        for (int32_t j = r.x + threadIdx.x; j < r.y; j += blockDim.x) check_entry(nb, r.x, j, ids_end, bad, unsorted);
+        int64_t lo, hi;
+        int32_t ids_end;
+        if (!row_span(rg, longs[k], n, nnz, voff, eoff, ng, lo, hi, ids_end)) continue;  // (flagged already)
+        for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) check_entry(nb, lo, j, ids_end, bad, unsorted);
     }
     const unsigned long long a = __ballot(bad), b = __ballot(unsorted);
     if ((threadIdx.x & 63) == 0) {
@@ -292,12 +326,11 @@ __global__ void k_u_label_init(const int2 *rg, int32_t n, unsigned long long *la
 // from the device (*nin), so several levels run between host read-backs;
 // frontier vertices with more than LONG_ROW entries go to lfront for
 // k_u_bfs_long.
-__device__ __forceinline__ void bfs_offer(unsigned long long *label, int32_t u, unsigned long long offer, int32_t *fout,
-                                          int32_t *nout)
+__device__ __forceinline__ bool bfs_claims(unsigned long long *label, int32_t u, unsigned long long offer)
 {
     const unsigned long long cur = label[u];
-    if (cur == ISOLATED || cur <= offer) return;
-    if (atomicMin(label + u, offer) == UNSET) fout[atomicAdd(nout, 1)] = u;
+    if (cur == ISOLATED || cur <= offer) return false;
+    return atomicMin(label + u, offer) == UNSET;
 }
 
 __global__ void k_u_bfs(const int2 *rg, const int32_t *nb, unsigned long long *label, const int32_t *fin,
@@ -305,19 +338,36 @@ __global__ void k_u_bfs(const int2 *rg, const int32_t *nb, unsigned long long *l
                         uint32_t dist)
 {
     const int gl = threadIdx.x & 7;
-    const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 3);
     const int32_t count = *nin;
-    for (int64_t k = (int64_t)blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3); k < count; k += groups) {
-        const int32_t v = fin[k];
-        const int2 r = rg[v];
-        if (r.y - r.x > LONG_ROW) {
-            if (gl == 0) lfront[atomicAdd(nlfront, 1)] = v;
-            continue;
+    const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 3);
+    // every lane of a wavefront runs the same iterations (appends are one
+    // atomic per wavefront)
+    for (int64_t k0 = (int64_t)blockIdx.x * (blockDim.x >> 3) + ((threadIdx.x & ~63) >> 3); k0 < count; k0 += wstride) {
+        const int64_t k = k0 + ((threadIdx.x & 63) >> 3);
+        int32_t v = -1;
+        int2 r = make_int2(0, 0);
+        if (k < count) {
+            v = fin[k];
+            r = rg[v];
         }
-        const unsigned long long offer = ((unsigned long long)(dist + 1) << 32) | (label[v] & 0xFFFFFFFFull);
-        for (int32_t j = r.x + gl; j < r.y; j += 8) {
-            const int32_t u = nb[j];
-            if (u != v) bfs_offer(label, u, offer, fout, nout);
+        const bool longrow = r.y - r.x > LONG_ROW;
+        const int32_t at = wave_slot(longrow && gl == 0, nlfront);
+        if (at >= 0) lfront[at] = v;
+        if (longrow) r = make_int2(0, 0);
+        const unsigned long long offer =
+            v >= 0 ? ((unsigned long long)(dist + 1) << 32) | (label[v] & 0xFFFFFFFFull) : 0ull;
+        int32_t steps = (r.y - r.x + 7) >> 3;
+        for (int o = 32; o > 0; o >>= 1) steps = max(steps, __shfl_xor(steps, o));
+        for (int32_t t = 0; t < steps; ++t) {
+            const int32_t j = r.x + t * 8 + gl;
+            bool claim = false;
+            int32_t u = -1;
+            if (j < r.y) {
+                u = nb[j];
+                claim = u != v && bfs_claims(label, u, offer);
+            }
+            const int32_t slot = wave_slot(claim, nout);
+            if (claim) fout[slot] = u;
         }
     }
 }
@@ -332,9 +382,16 @@ __global__ void k_u_bfs_long(const int2 *rg, const int32_t *nb, unsigned long lo
         const int32_t v = lfront[k];
         const int2 r = rg[v];
         const unsigned long long offer = ((unsigned long long)(dist + 1) << 32) | (label[v] & 0xFFFFFFFFull);
-        for (int32_t j = r.x + threadIdx.x; j < r.y; j += blockDim.x) {
-            const int32_t u = nb[j];
-            if (u != v) bfs_offer(label, u, offer, fout, nout);
+        for (int32_t j0 = r.x; j0 < r.y; j0 += blockDim.x) {
+            const int32_t j = j0 + threadIdx.x;
+            bool claim = false;
+            int32_t u = -1;
+            if (j < r.y) {
+                u = nb[j];
+                claim = u != v && bfs_claims(label, u, offer);
+            }
+            const int32_t slot = wave_slot(claim, nout);
+            if (claim) fout[slot] = u;
         }
     }
 }
@@ -747,7 +804,7 @@ __global__ void k_u_adjp(const int2 *rg, const int32_t *order, const int2 *rn, i
             if (jr < cv.x || jr >= cv.y) continue;  // (every entry of a symmetric CSR has its reverse)
             const int32_t base = rn[v].x;
             const int32_t at = base + spos[base + (jr - cv.x)];
-            adjp[at] = make_int2(adj[at].x, p);
+            reinterpret_cast<int32_t *>(adjp + at)[1] = p;  // (the id half is adj's, copied before)
         }
     }
 }
@@ -811,9 +868,25 @@ struct UploadState {
     int32_t *longs = nullptr;  // caller rows with more than LONG_ROW entries
     int32_t nlong = 0;
     int32_t *rev = nullptr;    // rev[j]: position of the reverse of entry j (symmetric CSRs, 32-bit keys)
+    int32_t *order = nullptr, *inv = nullptr;  // internal order: order[new] = old
+    hipStream_t wstream = nullptr;  // the weights' copy runs on its own stream and host thread
+    std::thread wthread;
+    hipError_t wresult = hipSuccess;
     std::vector<void *> temps;
+    // hipFree waits for the whole device, the weights' copy included: while
+    // that copy runs, freed scratch is kept here and released by the join.
+    std::vector<void *> deferred;
+    hipError_t join_weights()
+    {
+        if (wthread.joinable()) wthread.join();
+        for (void *p : deferred) (void)hipFree(p);
+        deferred.clear();
+        return wresult;
+    }
     ~UploadState()
     {
+        join_weights();
+        if (wstream) (void)hipStreamDestroy(wstream);
         for (void *p : temps) (void)hipFree(p);
     }
 };
@@ -853,7 +926,10 @@ void tmp_free(UploadState *u, void *p)
 {
     for (size_t k = 0; k < u->temps.size(); ++k)
         if (u->temps[k] == p) {
-            (void)hipFree(p);
+            if (u->wthread.joinable())
+                u->deferred.push_back(p);
+            else
+                (void)hipFree(p);
             u->temps.erase(u->temps.begin() + (long)k);
             return;
         }
@@ -1141,7 +1217,7 @@ int32_t locality_order(UploadState *u, int32_t *order, int32_t *inv)
 }  // namespace
 
 int32_t mm_upload_begin(mm_graph *g, const int32_t *ranges, const int32_t *nbrs, const float *w, const int32_t *voff,
-                        const int64_t *eoff, int32_t n_graphs, bool want_rev, UploadState **out)
+                        const int64_t *eoff, int32_t n_graphs, int relabel, UploadState **out)
 {
     *out = nullptr;
     UploadState *u = new (std::nothrow) UploadState();
@@ -1161,12 +1237,25 @@ int32_t mm_upload_begin(mm_graph *g, const int32_t *ranges, const int32_t *nbrs,
         return rc;
     if (w && (rc = tmp_alloc(u, &u->w, (size_t)g->nnz, "caller weights")) != MM_OK) return rc;
     if (nn) UP_TRY(hipMemcpyAsync(u->rg, ranges, sizeof(int2) * nn, hipMemcpyHostToDevice, s), "Unable to transfer graph data to device");
-    if (g->nnz) {
+    if (g->nnz)
         UP_TRY(hipMemcpyAsync(u->nb, nbrs, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyHostToDevice, s),
                "Unable to transfer graph data to device");
-        if (w)
-            UP_TRY(hipMemcpyAsync(u->w, w, sizeof(float) * (size_t)g->nnz, hipMemcpyHostToDevice, s),
-                   "Unable to transfer graph data to device");
+    if (w && g->nnz) {
+        // The weights are needed first by the symmetry test: their copy runs
+        // beside the checks and the locality order, on a host thread and
+        // stream of its own, after the rows' copy (two host copies at once
+        // share the bus and take longer than one after the other).
+        UP_TRY(hipStreamSynchronize(s), "Unable to transfer graph data to device");
+        UP_TRY(hipStreamCreateWithFlags(&u->wstream, hipStreamNonBlocking), "hipStreamCreate");
+        const int dev = g->device;
+        const size_t bytes = sizeof(float) * (size_t)g->nnz;
+        UploadState *up = u;
+        u->wthread = std::thread([up, dev, w, bytes] {
+            hipError_t e = hipSetDevice(dev);
+            if (e == hipSuccess) e = hipMemcpyAsync(up->w, w, bytes, hipMemcpyHostToDevice, up->wstream);
+            if (e == hipSuccess) e = hipStreamSynchronize(up->wstream);
+            up->wresult = e;
+        });
     }
     long long *d_eoff = nullptr;
     if (voff && n_graphs > 0) {
@@ -1180,37 +1269,42 @@ int32_t mm_upload_begin(mm_graph *g, const int32_t *ranges, const int32_t *nbrs,
                "Unable to transfer graph data to device");
     }
     mm_upload_mark("copy to device", s);
-    if (d_eoff && nn) {
-        // a batch's row ranges shift by their graph's first entry (its ids stay
-        // local until they are checked)
-        hipLaunchKernelGGL(k_u_rebase_rows, dim3(grid_for((int64_t)nn)), dim3(UB), 0, s, u->rg, g->n, u->voff, d_eoff,
-                           n_graphs);
-        UP_TRY(hipGetLastError(), "batch rebase");
-    }
-    // Neighbour ids and ascending rows (ids still local to their graph).
+    // Ranges, ids and ascending rows (a batch's ranges and ids still local).
     int32_t *flags = nullptr;
-    if ((rc = tmp_alloc(u, &flags, 4, "check flags")) != MM_OK || (rc = tmp_alloc(u, &u->longs, nn, "long rows")) != MM_OK)
+    unsigned long long *hist = nullptr;
+    if ((rc = tmp_alloc(u, &flags, 4, "check flags")) != MM_OK || (rc = tmp_alloc(u, &u->longs, nn, "long rows")) != MM_OK ||
+        (rc = tmp_alloc(u, &hist, 33, "degree histogram")) != MM_OK)
         return rc;
     UP_TRY(hipMemsetAsync(flags, 0, 4 * sizeof(int32_t), s), "check flags");
+    UP_TRY(hipMemsetAsync(hist, 0, 33 * sizeof(unsigned long long), s), "degree histogram");
     bool sorted = true;
     if (nn) {
-        hipLaunchKernelGGL(k_u_check, dim3(grid_for((int64_t)nn * 8)), dim3(UB), 0, s, u->rg, u->nb, g->n, u->voff, ng,
-                           flags, u->longs, flags + 2);
+        hipLaunchKernelGGL(k_u_check, dim3(grid_for((int64_t)nn * 8)), dim3(UB), 0, s, u->rg, u->nb, g->n, g->nnz, u->voff,
+                           d_eoff, ng, flags, u->longs, hist);
         UP_TRY(hipGetLastError(), "k_u_check");
         int32_t h_flags[4];
+        unsigned long long h_hist[33];
+        UP_TRY(hipMemcpyAsync(h_hist, hist, sizeof(h_hist), hipMemcpyDeviceToHost, s), "degree histogram");
         UP_TRY(copy_on(s, h_flags, flags, sizeof(h_flags), hipMemcpyDeviceToHost), "check flags");
         u->nlong = h_flags[2];
-        if (u->nlong > 0) {
-            hipLaunchKernelGGL(k_u_check_long, dim3(std::min(u->nlong, 4096)), dim3(UB), 0, s, u->rg, u->nb, g->n, u->voff,
-                               ng, flags, u->longs, u->nlong);
+        g->max_degree = h_flags[3];
+        for (int k = 0; k < 33; ++k) g->deg_hist[k] = (int64_t)h_hist[k];
+        if (u->nlong > 0 && !h_flags[0]) {
+            hipLaunchKernelGGL(k_u_check_long, dim3(std::min(u->nlong, 4096)), dim3(UB), 0, s, u->rg, u->nb, g->n, g->nnz,
+                               u->voff, d_eoff, ng, flags, u->longs, u->nlong);
             UP_TRY(hipGetLastError(), "k_u_check_long");
             UP_TRY(copy_on(s, h_flags, flags, sizeof(h_flags), hipMemcpyDeviceToHost), "check flags");
         }
         if (h_flags[0]) return mm::set_error(MM_ERR_INVALID, "malformed CSR: range or neighbour id out of bounds");
         sorted = !h_flags[1];
     }
+    g->mean_degree = g->n > 0 ? (double)g->nnz / (double)g->n : 0.0;
+    tmp_free(u, hist);
     if (d_eoff) {
-        // one CSR: ids shift by their graph's first vertex
+        // one CSR: row ranges shift by their graph's first entry, ids by its first vertex
+        if (nn)
+            hipLaunchKernelGGL(k_u_rebase_rows, dim3(grid_for((int64_t)nn)), dim3(UB), 0, s, u->rg, g->n, u->voff, d_eoff,
+                               n_graphs);
         if (g->nnz)
             hipLaunchKernelGGL(k_u_rebase_ids, dim3(grid_for(g->nnz)), dim3(UB), 0, s, u->nb, g->nnz, u->voff, d_eoff,
                                n_graphs);
@@ -1218,7 +1312,30 @@ int32_t mm_upload_begin(mm_graph *g, const int32_t *ranges, const int32_t *nbrs,
         UP_TRY(hipStreamSynchronize(s), "batch rebase");
         tmp_free(u, d_eoff);
     }
+    // 32-bit proposal keys hold (position << L) | local id, L = bits of the
+    // largest local id: possible only while the largest degree leaves room
+    int32_t max_loc = g->n > 0 ? g->n - 1 : 0;
+    if (voff && n_graphs > 0) {
+        max_loc = 0;
+        for (int32_t k = 0; k < n_graphs; ++k) max_loc = std::max(max_loc, voff[k + 1] - voff[k] - 1);
+    }
+    int locbits = 1;
+    while (locbits < 31 && ((int64_t)1 << locbits) <= (int64_t)max_loc) ++locbits;
+    const bool want_rev = w && locbits < 31 && (int64_t)g->max_degree < ((int64_t)1 << (32 - locbits));
     mm_upload_mark("id checks", s);
+    // The internal order (reads rows and ids only).
+    if ((rc = tmp_alloc(u, &u->order, nn, "vertex order")) != MM_OK || (rc = tmp_alloc(u, &u->inv, nn, "vertex order")) != MM_OK)
+        return rc;
+    g->relabelled = relabel != 0 && g->n > 1;
+    if (g->relabelled) {
+        if ((rc = locality_order(u, u->order, u->inv)) != MM_OK) return rc;
+    } else if (nn) {
+        hipLaunchKernelGGL(k_u_identity, dim3(grid_for((int64_t)nn)), dim3(UB), 0, s, g->n, u->order, u->inv);
+        UP_TRY(hipGetLastError(), "k_u_identity");
+    }
+    mm_upload_mark("order", s);
+    UP_TRY(u->join_weights(), "Unable to transfer graph weights to device");
+    mm_upload_mark("weights copy (overlapped)", nullptr);
     g->push_ok_struct = 0;
     g->push_ok_weights = 0;
     if (sorted && nn) {
@@ -1270,24 +1387,14 @@ void mm_upload_mark(const char *phase, hipStream_t s)
     last = now;
 }
 
-int32_t mm_upload_finish(mm_graph *g, UploadState *u, int relabel, bool align_rows, bool heads)
+int32_t mm_upload_finish(mm_graph *g, UploadState *u, bool align_rows, bool heads)
 {
     hipStream_t s = g->stream;
     const int32_t n = g->n;
     const size_t nn = (size_t)n;
     const int32_t ng = u->voff ? (int32_t)u->h_voff.size() - 1 : 1;
     int32_t rc;
-    int32_t *order = nullptr, *inv = nullptr;
-    if ((rc = tmp_alloc(u, &order, nn, "vertex order")) != MM_OK || (rc = tmp_alloc(u, &inv, nn, "vertex order")) != MM_OK)
-        return rc;
-    g->relabelled = relabel != 0 && n > 1;
-    if (g->relabelled) {
-        if ((rc = locality_order(u, order, inv)) != MM_OK) return rc;
-    } else if (n > 0) {
-        hipLaunchKernelGGL(k_u_identity, dim3(grid_for(n)), dim3(UB), 0, s, n, order, inv);
-        UP_TRY(hipGetLastError(), "k_u_identity");
-    }
-    mm_upload_mark("order", s);
+    int32_t *order = u->order, *inv = u->inv;
     // Row offsets in the internal order.
     if ((rc = mm_graph_alloc(g, (void **)&g->d_ranges, sizeof(int2) * nn, "neighbour ranges")) != MM_OK) return rc;
     int64_t nnz_dev = 0;

@@ -40,19 +40,6 @@ def test_no_device_here_is_reported_not_faked():
     assert e.value.code == mmamd.MM_ERR_HIP
 
 
-def test_malformed_ranges_are_rejected_on_the_host():
-    """Row ranges are checked on the host before any device is touched (the
-    neighbour ids on the device: tests/test_gpu_parity.py)."""
-    bad_range = CSR(np.array([[0, 3], [3, 2]], np.int32), np.array([1, 0, 1], np.int32), None)
-    with pytest.raises(mmamd.MatchError) as e:
-        mmamd.Graph(bad_range)
-    assert e.value.code == mmamd.MM_ERR_INVALID and "out of bounds" in str(e.value)
-    past_end = CSR(np.array([[0, 1], [1, 4]], np.int32), np.array([1, 0, 1], np.int32), None)
-    with pytest.raises(mmamd.MatchError) as e:
-        mmamd.Graph(past_end)
-    assert e.value.code == mmamd.MM_ERR_INVALID
-
-
 def coords(n, seed):
     """The RGG point coordinates, restated with numpy (SplitMix64 finaliser)."""
     m64 = (1 << 64) - 1

@@ -261,10 +261,14 @@ def test_batch_equals_single_graph_path():
         assert res.rounds == max(rounds)
 
 
-def test_malformed_ids_are_rejected_on_the_device():
-    """Neighbour ids outside the graph (or a batch graph's local range) are
-    rejected before any kernel reads by them; the handle is not created."""
-    for bad in (CSR.from_offsets([0, 1, 2], np.array([1, 5], np.int32)),
+def test_malformed_csr_is_rejected_on_the_device():
+    """Row ranges outside the entries and neighbour ids outside the graph (or a
+    batch graph's local range) are rejected before any kernel reads by them;
+    the handle is not created."""
+    for bad in (CSR(np.array([[0, 3], [3, 2]], np.int32), np.array([1, 0, 1], np.int32), None),   # end < begin
+                CSR(np.array([[0, 1], [1, 4]], np.int32), np.array([1, 0, 1], np.int32), None),   # past the entries
+                CSR(np.array([[-1, 1], [1, 2]], np.int32), np.array([1, 0], np.int32), None),     # negative begin
+                CSR.from_offsets([0, 1, 2], np.array([1, 5], np.int32)),
                 CSR.from_offsets([0, 1, 2], np.array([1, -1], np.int32)),
                 # a row of more than 256 entries (checked by a whole block) holding id 7
                 CSR.from_offsets([0, 300, 301], np.concatenate([np.ones(299), [7], [0]]).astype(np.int32))):
@@ -274,6 +278,9 @@ def test_malformed_ids_are_rejected_on_the_device():
     good = CSR.from_offsets([0, 1, 2], np.array([1, 0], np.int32))
     with pytest.raises(mmamd.MatchError) as e:  # graph 1's id 1 is local: past its single vertex
         Batch([good, CSR.from_offsets([0, 1], np.array([1], np.int32))])
+    assert e.value.code == mmamd.MM_ERR_INVALID
+    with pytest.raises(mmamd.MatchError) as e:  # graph 1's range passes its own single entry
+        Batch([good, CSR(np.array([[0, 2]], np.int32), np.array([0], np.int32), None)])
     assert e.value.code == mmamd.MM_ERR_INVALID
     assert Graph(good).match(seeds=SEEDS).match.tolist() == [4, 4]
 
