@@ -466,6 +466,7 @@ def run_batch(args, cpu: bool) -> dict:
         "kernel_ms": kms,
         "matched_vertices": chk["size"], "matching_weight": chk["weight"], "valid": bool(chk["valid"]),
         "graphs_valid": int(gv.sum()), "still_matchable": chk["matchable"], "device_check_ms": round(min(check_ms), 2),
+        "device_check_first_ms": round(check_ms[0], 2),
         "graphs_checked_against_single_path": len(pick), "graphs_differing": bad, "autotune": tune,
     }
     b.close()

@@ -35,6 +35,8 @@ struct mm_graph {
     int2 *d_ranges = nullptr;
     int32_t *d_nbrs = nullptr;
     int2 *d_adj = nullptr;
+    float *d_wcsr = nullptr;  // weighted graphs whose rows may repeat an id: weights in CSR order at the internal
+                              // positions (the result check's getWeight); null otherwise
     int2 *d_adjp = nullptr;   // adj with each weight replaced by the edge's position in the other end's
                               // sorted row (32-bit proposal keys; PUSH graphs that fit them)
     int locbits = 0;          // bits of local id in a 32-bit proposal key
@@ -123,6 +125,9 @@ void mm_upload_free(UploadState *u);
 // Enqueues the copy of the internal match array into the caller's vertex order
 // (relabelled graphs only; a no-op otherwise).
 void mm_graph_enqueue_unpermute(mm_graph *g);
+// The static scratch of the no-allocation check on `device` (the current
+// device): its device address and size (mm_scratch.hip).
+int32_t mm_noalloc_scratch(int device, char **out, size_t *bytes);
 // Rounds a maximal run should run next, from the active counts of its last
 // two rounds (before, now); the chunk length between counter read-backs.
 int predicted_rounds(int64_t before, int64_t now);

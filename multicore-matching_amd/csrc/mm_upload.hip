@@ -666,6 +666,18 @@ __global__ void k_u_loc(const int32_t *order, int32_t n, const int32_t *voff, in
     }
 }
 
+// CSR-order weights at the internal positions (rows that may repeat an id), 8 lanes per row.
+__global__ void k_u_wcsr(const int2 *rg, const float *w, const int32_t *order, const int2 *rn, int32_t n, float *out)
+{
+    const int gl = threadIdx.x & 7;
+    const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 3);
+    for (int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 3) + (threadIdx.x >> 3); i < n; i += groups) {
+        const int2 r = rg[order[i]];
+        const int32_t dst = rn[i].x;
+        for (int32_t k = gl; k < r.y - r.x; k += 8) out[dst + k] = w[r.x + k];
+    }
+}
+
 // Renumbered CSR-order rows, 8 lanes per row.
 __global__ void k_u_nbrs(const int2 *rg, const int32_t *nb, const int32_t *order, const int32_t *inv, const int2 *rn,
                          int32_t n, int32_t *out)
@@ -867,6 +879,7 @@ struct UploadState {
     std::vector<int32_t> h_voff;
     int32_t *longs = nullptr;  // caller rows with more than LONG_ROW entries
     int32_t nlong = 0;
+    bool strict = true;        // every row strictly ascending (no repeated id)
     int32_t *rev = nullptr;    // rev[j]: position of the reverse of entry j (symmetric CSRs, 32-bit keys)
     int32_t *order = nullptr, *inv = nullptr;  // internal order: order[new] = old
     hipStream_t wstream = nullptr;  // the weights' copy runs on its own stream and host thread
@@ -1297,6 +1310,7 @@ int32_t mm_upload_begin(mm_graph *g, const int32_t *ranges, const int32_t *nbrs,
         }
         if (h_flags[0]) return mm::set_error(MM_ERR_INVALID, "malformed CSR: range or neighbour id out of bounds");
         sorted = !h_flags[1];
+        u->strict = sorted;
     }
     g->mean_degree = g->n > 0 ? (double)g->nnz / (double)g->n : 0.0;
     tmp_free(u, hist);
@@ -1446,6 +1460,15 @@ int32_t mm_upload_finish(mm_graph *g, UploadState *u, bool align_rows, bool head
         hipLaunchKernelGGL(k_u_nbrs, dim3(grid_for((int64_t)n * 8)), dim3(UB), 0, s, u->rg, u->nb, order, inv,
                            g->d_ranges, n, g->d_nbrs);
         UP_TRY(hipGetLastError(), "k_u_nbrs");
+    }
+    if (g->weighted && !u->strict && n > 0 && nnz_dev > 0) {
+        // A row may hold one id twice with different weights: the result
+        // check's getWeight reads the weight at the CSR position it stops at.
+        if ((rc = mm_graph_alloc(g, (void **)&g->d_wcsr, sizeof(float) * (size_t)nnz_dev, "CSR-order weights")) != MM_OK)
+            return rc;
+        hipLaunchKernelGGL(k_u_wcsr, dim3(grid_for((int64_t)n * 8)), dim3(UB), 0, s, u->rg, u->w, order, g->d_ranges, n,
+                           g->d_wcsr);
+        UP_TRY(hipGetLastError(), "k_u_wcsr");
     }
     mm_upload_mark("ids and rows", s);
     int32_t max_loc = n > 0 ? n - 1 : 0;

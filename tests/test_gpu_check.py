@@ -87,3 +87,30 @@ def test_batch_device_check_per_graph_equals_oracle():
         assert tot["weight"] == sum(e["weight"] for e in exp)
         assert tot["valid"] == int(all(e["valid"] for e in exp))
     b.close()
+
+
+@pytest.mark.parametrize("keep", [False, True])
+def test_device_check_weight_is_the_csr_first_entry(keep):
+    """getWeight takes the weight at the first CSR position whose neighbour
+    shares the value: a repeated neighbour's lighter copy when it comes first,
+    a self-loop when it precedes the partner."""
+    rows = [[(1, 0.25), (1, 0.75), (2, 0.5)],          # 0: neighbour 1 twice, lighter copy first
+            [(0, 0.75), (0, 0.25), (4, 0.5)],          # 1: heavier copy first
+            [(2, 0.125), (3, 0.5), (0, 0.5)],          # 2: self-loop ahead of partner 3
+            [(2, 0.5), (5, 0.375)],
+            [(1, 0.5), (5, 0.625)],
+            [(3, 0.375), (4, 0.625)]]
+    offsets = np.cumsum([0] + [len(r) for r in rows])
+    nb = [u for r in rows for u, _ in r]
+    w = [x for r in rows for _, x in r]
+    csr = mmamd.CSR.from_offsets(offsets, nb, w)
+    g = Graph(csr, keep_order=keep)
+    arrays = [np.array([4, 4, 6, 6, 8, 8], np.int32),    # {0,1} {2,3} {4,5}
+              np.array([4, 4, 6, 6, 0, 1], np.int32),
+              np.array([4, 4, 4, 6, 6, 2], np.int32),    # a group of three, a pair that is no edge
+              np.array([4, 4, 9, 9, 30, 30], np.int32)]  # a value past the end
+    for m in arrays:
+        g.set_match(m)
+        exp = oracle_check(csr, m)
+        assert_check(g.check(), exp, ("keep", keep, m.tolist()))
+    g.close()
