@@ -681,8 +681,10 @@ __device__ T block_sum(T x)
 
 // Select of round 1 (match starts at 0 everywhere, src/matchgpu.cu:486).
 // Block b colours the ids of bucket b and files them into round 1's lists.
+// Held to 64 VGPRs (8 wavefronts per SIMD): two blocks of 1024 threads share
+// a CU, so all the buckets' blocks of a 2^22-vertex graph run at once.
 template <bool PUSH>
-__global__ void k_init_select(Dev d)
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per_eu(8))) void k_init_select(Dev d)
 {
     __shared__ int32_t s_cnt[2];
     const int parity = 1, b = bucket_of_block(d);
@@ -704,7 +706,7 @@ __global__ void k_init_select(Dev d)
     // row ranges serve the hub test, the round's lists and the degree sums
     // only: without lists, hubs or sums they are not read.
     const bool need_rows = !d.listless1 || !d.no_hubs || d.count_nnz;
-    constexpr int IU = 4;
+    constexpr int IU = 2;
     for (int64_t base = lo + (threadIdx.x & ~63); base < hi; base += (int64_t)blockDim.x * IU) {
         int32_t vl[IU];
         int2 rg[IU];
@@ -1485,7 +1487,6 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     if ((rc = dev_alloc(g, &g->d_match, nn, "match")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_req, nn, "requests")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_st, nn, "state")) != MM_OK) return fail(rc);
-    if (g->push_ok_struct && (rc = dev_alloc(g, &g->d_resp, 2 * nn, "proposal keys")) != MM_OK) return fail(rc);
     // Buckets: at most one per block of the smallest block size (64 threads).
     g->nb_max = std::max(1, g->num_cu * 32);
     g->list_cap = (int64_t)n + g->nb_max;
@@ -1507,6 +1508,8 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
         return fail(mm::set_error(MM_ERR_NOMEM, "pinned host allocation failed"));
     mm_upload_mark("matching buffers", nullptr);
     if ((rc = mm_upload_finish(g, up, align_rows, heads)) != MM_OK) return fail(rc);
+    // (the symmetry test, inside the finish, decides whether proposals are pushed)
+    if (g->push_ok_struct && (rc = dev_alloc(g, &g->d_resp, 2 * nn, "proposal keys")) != MM_OK) return fail(rc);
     *out = g;
     return MM_OK;
 }

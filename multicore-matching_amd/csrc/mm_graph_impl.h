@@ -109,11 +109,12 @@ int32_t mm_graph_alloc(mm_graph *g, void **p, size_t bytes, const char *what);
 // (MM_ERR_INVALID when one leaves its graph) and records the degree
 // statistics, rebases a batch's per-graph rows into one CSR by voff / eoff,
 // computes the internal order (relabel: the locality order, else the
-// caller's) and, when every row is strictly ascending, tests the symmetry that
-// proposal atomics need (sets push_ok_*; also keeps every entry's reverse
-// when the graph can take 32-bit keys); finish builds the rows, sorted rows,
-// key positions, heads and caller-order pairs.  mm_upload_free releases the
-// temporaries.
+// caller's); finish builds the rows and, when every row is strictly
+// ascending, tests the symmetry that proposal atomics need (sets push_ok_*,
+// so d_resp is allocated after it; also keeps every entry's reverse when the
+// graph can take 32-bit keys), then the sorted rows, key positions, heads and
+// caller-order pairs.  The weights' copy runs beside begin and the rows, up to
+// the symmetry test.  mm_upload_free releases the temporaries.
 struct UploadState;
 // Upload phase times on stderr when the environment sets MM_UPLOAD_TIMING:
 // the time since the previous mark (stream s drained first; null: none).

@@ -73,6 +73,29 @@ __device__ __forceinline__ int32_t wave_slot(bool pred, int32_t *counter)
     return pred ? base + (int32_t)__popcll(m & ((1ull << lane) - 1ull)) : -1;
 }
 
+// Block-wide sum / max / or of one value per thread, posted to *dst with one
+// global atomic per block (kernels of many small blocks otherwise queue
+// hundreds of thousands of atomics on a single address).  Reached by all
+// threads of the block.
+template <typename T>
+__device__ __forceinline__ T block_reduce(T x, int op)  // op 0: sum, 1: max, 2: or; result in thread 0
+{
+    __shared__ T s_part[UB / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        const T y = __shfl_xor(x, o);
+        x = op == 0 ? x + y : (op == 1 ? (x > y ? x : y) : (x | y));
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+            const T y = s_part[k];
+            x = op == 0 ? x + y : (op == 1 ? (x > y ? x : y) : (x | y));
+        }
+    return x;
+}
+
 __device__ __forceinline__ int32_t graph_of(const int32_t *voff, int32_t ng, int32_t v)
 {
     int32_t lo = 0, hi = ng - 1;
@@ -180,11 +203,11 @@ __global__ __launch_bounds__(UB) void k_u_check(const int2 *rg, const int32_t *n
         }
         for (int64_t j = lo + gl; j < hi; j += 8) check_entry(nb, lo, j, ids_end, bad, unsorted);
     }
-    for (int o = 32; o > 0; o >>= 1) maxdeg = max(maxdeg, __shfl_xor(maxdeg, o));
-    const unsigned long long a = __ballot(bad), b = __ballot(unsorted);
-    if ((threadIdx.x & 63) == 0) {
-        if (a) atomicOr(flags, 1);
-        if (b) atomicOr(flags + 1, 1);
+    maxdeg = block_reduce(maxdeg, 1);
+    const int32_t anybad = block_reduce(bad | (unsorted << 1), 2);
+    if (threadIdx.x == 0) {
+        if (anybad & 1) atomicOr(flags, 1);
+        if (anybad & 2) atomicOr(flags + 1, 1);
         if (maxdeg) atomicMax(flags + 3, maxdeg);
     }
     __syncthreads();
@@ -254,16 +277,14 @@ __device__ __forceinline__ void sym_entry(const int2 *rg, const int32_t *nb, con
 __device__ __forceinline__ void sym_flush(unsigned long long down, unsigned long long up, int asym, int wasym,
                                           unsigned long long *counts, int32_t *flags)
 {
-    for (int o = 32; o > 0; o >>= 1) {
-        down += __shfl_xor(down, o);
-        up += __shfl_xor(up, o);
-    }
-    const unsigned long long a = __ballot(asym), b = __ballot(wasym);
-    if ((threadIdx.x & 63) == 0) {
+    down = block_reduce(down, 0);
+    up = block_reduce(up, 0);
+    const int32_t f = block_reduce(asym | (wasym << 1), 2);
+    if (threadIdx.x == 0) {
         if (down) atomicAdd(counts, down);
         if (up) atomicAdd(counts + 1, up);
-        if (a) atomicOr(flags, 1);
-        if (b) atomicOr(flags + 1, 1);
+        if (f & 1) atomicOr(flags, 1);
+        if (f & 2) atomicOr(flags + 1, 1);
     }
 }
 
@@ -333,10 +354,16 @@ __device__ __forceinline__ bool bfs_claims(unsigned long long *label, int32_t u,
     return atomicMin(label + u, offer) == UNSET;
 }
 
-__global__ void k_u_bfs(const int2 *rg, const int32_t *nb, unsigned long long *label, const int32_t *fin,
-                        const int32_t *nin, int32_t *fout, int32_t *nout, int32_t *lfront, int32_t *nlfront,
-                        uint32_t dist)
+constexpr int BFS_BUF = 4096;  // a block's next-frontier entries gathered in LDS before one global append
+
+__global__ __launch_bounds__(UB) void k_u_bfs(const int2 *rg, const int32_t *nb, unsigned long long *label,
+                                              const int32_t *fin, const int32_t *nin, int32_t *fout, int32_t *nout,
+                                              int32_t *lfront, int32_t *nlfront, uint32_t dist)
 {
+    __shared__ int32_t s_buf[BFS_BUF];
+    __shared__ int32_t s_n, s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
     const int gl = threadIdx.x & 7;
     const int32_t count = *nin;
     const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 3);
@@ -366,10 +393,18 @@ __global__ void k_u_bfs(const int2 *rg, const int32_t *nb, unsigned long long *l
                 u = nb[j];
                 claim = u != v && bfs_claims(label, u, offer);
             }
-            const int32_t slot = wave_slot(claim, nout);
-            if (claim) fout[slot] = u;
+            // the block's buffer first; the global list once it is full
+            const int32_t at = wave_slot(claim, &s_n);
+            if (claim && at < BFS_BUF) s_buf[at] = u;
+            const int32_t slot = wave_slot(claim && at >= BFS_BUF, nout);
+            if (slot >= 0) fout[slot] = u;
         }
     }
+    __syncthreads();
+    const int32_t kept = min(s_n, BFS_BUF);
+    if (threadIdx.x == 0) s_base = kept ? atomicAdd(nout, kept) : 0;
+    __syncthreads();
+    for (int32_t k = threadIdx.x; k < kept; k += blockDim.x) fout[s_base + k] = s_buf[k];
 }
 
 // The same for the level's frontier vertices with more than LONG_ROW entries,
@@ -418,8 +453,8 @@ __global__ void k_u_seed_mark(const int2 *rg, const int32_t *nb, const unsigned 
         mark[v] = m;
         c += m;
     }
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+    c = block_reduce(c, 0);
+    if (threadIdx.x == 0 && c) atomicAdd(count, c);
 }
 
 __global__ void k_u_seed_apply(const uint8_t *mark, int32_t n, unsigned long long *label, int32_t *front, int32_t *fcount)
@@ -710,16 +745,36 @@ __global__ void k_u_gather_ranges(const int32_t *list, int32_t k, const int2 *rn
 }
 
 // Rows classified by length: [0] up to 16 entries, [1] up to 64, [2] longer.
-__global__ void k_u_classify(const int2 *rn, int32_t n, int32_t *lists, int32_t cap, int32_t *counts)
+// A block takes a contiguous chunk of rows: counts them per class in LDS,
+// reserves its room with one global atomic per class, then fills it.
+__global__ __launch_bounds__(UB) void k_u_classify(const int2 *rn, int32_t n, int32_t *lists, int32_t cap,
+                                                   int32_t *counts)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63); i0 < n; i0 += stride) {
-        const int64_t i = i0 + (threadIdx.x & 63);
-        const int32_t deg = i < n ? rn[i].y - rn[i].x : 0;
-        const int c = deg == 0 ? -1 : (deg <= 16 ? 0 : (deg <= 64 ? 1 : 2));
+    __shared__ int32_t s_cnt[3], s_base[3];
+    if (threadIdx.x < 3) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t per = ((int64_t)n + gridDim.x - 1) / gridDim.x;
+    const int64_t i0 = (int64_t)blockIdx.x * per, i1 = min((int64_t)n, i0 + per);
+    auto class_of = [&](int64_t i) {
+        const int32_t deg = rn[i].y - rn[i].x;
+        return deg == 0 ? -1 : (deg <= 16 ? 0 : (deg <= 64 ? 1 : 2));
+    };
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const int c = class_of(i);
+        if (c >= 0) atomicAdd(s_cnt + c, 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        s_base[threadIdx.x] = s_cnt[threadIdx.x] ? atomicAdd(counts + threadIdx.x, s_cnt[threadIdx.x]) : 0;
+        s_cnt[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    for (int64_t b0 = i0; b0 < i1; b0 += blockDim.x) {
+        const int64_t i = b0 + threadIdx.x;
+        const int c = i < i1 ? class_of(i) : -1;
         for (int k = 0; k < 3; ++k) {
-            const int32_t at = wave_slot(c == k, counts + k);
-            if (c == k) lists[(size_t)k * cap + at] = (int32_t)i;
+            const int32_t at = wave_slot(c == k, s_cnt + k);
+            if (c == k) lists[(size_t)k * cap + s_base[k] + at] = (int32_t)i;
         }
     }
 }
@@ -880,6 +935,7 @@ struct UploadState {
     int32_t *longs = nullptr;  // caller rows with more than LONG_ROW entries
     int32_t nlong = 0;
     bool strict = true;        // every row strictly ascending (no repeated id)
+    bool want_rev = false;     // the graph could take 32-bit keys: the symmetry test keeps the reverses
     int32_t *rev = nullptr;    // rev[j]: position of the reverse of entry j (symmetric CSRs, 32-bit keys)
     int32_t *order = nullptr, *inv = nullptr;  // internal order: order[new] = old
     hipStream_t wstream = nullptr;  // the weights' copy runs on its own stream and host thread
@@ -951,6 +1007,13 @@ void tmp_free(UploadState *u, void *p)
 int grid_for(int64_t items, int per_block = UB)
 {
     return (int)std::max<int64_t>(1, std::min<int64_t>((items + per_block - 1) / per_block, 65536));
+}
+
+// Grid of a grid-stride kernel that posts block totals with global atomics:
+// about 8 blocks per CU, enough to fill the chip.
+int grid_red(const UploadState *u, int64_t items)
+{
+    return std::min(grid_for(items), std::max(8, u->g->num_cu * 8));
 }
 
 int32_t read_i32(const int32_t *d, hipStream_t s, int32_t *out)
@@ -1056,7 +1119,7 @@ int32_t locality_order(UploadState *u, int32_t *order, int32_t *inv)
         if ((rc = grow_regions(u, label, fa, fb, lf, lc, count)) != MM_OK) return rc;
         // vertices no search reached seed regions of their own
         UP_TRY(hipMemsetAsync(ctr, 0, 2 * sizeof(int32_t), s), "upload counters");
-        hipLaunchKernelGGL(k_u_seed_mark, dim3(grid_for(n)), dim3(UB), 0, s, u->rg, u->nb, label, n, mark, ctr);
+        hipLaunchKernelGGL(k_u_seed_mark, dim3(grid_red(u, n)), dim3(UB), 0, s, u->rg, u->nb, label, n, mark, ctr);
         UP_TRY(hipGetLastError(), "k_u_seed_mark");
         int32_t marked = 0;
         if ((rc = read_i32(ctr, s, &marked)) != MM_OK) return rc;
@@ -1229,6 +1292,57 @@ int32_t locality_order(UploadState *u, int32_t *order, int32_t *inv)
 
 }  // namespace
 
+namespace {
+
+// The symmetry test (when every row ascends): push_ok_struct / push_ok_weights,
+// and each entry's reverse when the graph can take 32-bit keys.  Needs the
+// weights, so it joins their copy.
+int32_t symmetry_test(UploadState *u)
+{
+    mm_graph *g = u->g;
+    hipStream_t s = g->stream;
+    const size_t nn = (size_t)g->n;
+    int32_t rc;
+    UP_TRY(u->join_weights(), "Unable to transfer graph weights to device");
+    mm_upload_mark("weights copy (overlapped)", nullptr);
+    g->push_ok_struct = 0;
+    g->push_ok_weights = 0;
+    if (!u->strict || !nn) return MM_OK;
+    unsigned long long *counts = nullptr;
+    int32_t *flags = nullptr;
+    if ((rc = tmp_alloc(u, &counts, 2, "symmetry counts")) != MM_OK || (rc = tmp_alloc(u, &flags, 2, "symmetry flags")) != MM_OK)
+        return rc;
+    if (u->want_rev && g->nnz) {
+        if ((rc = tmp_alloc(u, &u->rev, (size_t)g->nnz, "reverse positions")) != MM_OK) return rc;
+        UP_TRY(hipMemsetAsync(u->rev, 0xFF, sizeof(int32_t) * (size_t)g->nnz, s), "reverse positions");  // -1: none
+    }
+    UP_TRY(hipMemsetAsync(counts, 0, 2 * sizeof(unsigned long long), s), "symmetry counts");
+    UP_TRY(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), s), "symmetry flags");
+    hipLaunchKernelGGL(k_u_sym, dim3(grid_red(u, (int64_t)nn * 8)), dim3(UB), 0, s, u->rg, u->nb, u->w, g->n, u->rev,
+                       counts, flags);
+    if (u->nlong > 0)
+        hipLaunchKernelGGL(k_u_sym_long, dim3(std::min(u->nlong, 4096)), dim3(UB), 0, s, u->rg, u->nb, u->w, u->longs,
+                           u->nlong, u->rev, counts, flags);
+    UP_TRY(hipGetLastError(), "k_u_sym");
+    unsigned long long h_counts[2];
+    int32_t h_flags[2];
+    UP_TRY(hipMemcpyAsync(h_counts, counts, sizeof(h_counts), hipMemcpyDeviceToHost, s), "symmetry counts");
+    UP_TRY(copy_on(s, h_flags, flags, sizeof(h_flags), hipMemcpyDeviceToHost), "symmetry flags");
+    const bool sym = !h_flags[0] && h_counts[0] == h_counts[1];
+    g->push_ok_struct = sym;
+    g->push_ok_weights = sym && !h_flags[1];
+    tmp_free(u, counts);
+    tmp_free(u, flags);
+    if (!g->push_ok_weights && u->rev) {
+        tmp_free(u, u->rev);
+        u->rev = nullptr;
+    }
+    mm_upload_mark("symmetry test", s);
+    return MM_OK;
+}
+
+}  // namespace
+
 int32_t mm_upload_begin(mm_graph *g, const int32_t *ranges, const int32_t *nbrs, const float *w, const int32_t *voff,
                         const int64_t *eoff, int32_t n_graphs, int relabel, UploadState **out)
 {
@@ -1292,7 +1406,7 @@ int32_t mm_upload_begin(mm_graph *g, const int32_t *ranges, const int32_t *nbrs,
     UP_TRY(hipMemsetAsync(hist, 0, 33 * sizeof(unsigned long long), s), "degree histogram");
     bool sorted = true;
     if (nn) {
-        hipLaunchKernelGGL(k_u_check, dim3(grid_for((int64_t)nn * 8)), dim3(UB), 0, s, u->rg, u->nb, g->n, g->nnz, u->voff,
+        hipLaunchKernelGGL(k_u_check, dim3(grid_red(u, (int64_t)nn * 8)), dim3(UB), 0, s, u->rg, u->nb, g->n, g->nnz, u->voff,
                            d_eoff, ng, flags, u->longs, hist);
         UP_TRY(hipGetLastError(), "k_u_check");
         int32_t h_flags[4];
@@ -1348,40 +1462,8 @@ int32_t mm_upload_begin(mm_graph *g, const int32_t *ranges, const int32_t *nbrs,
         UP_TRY(hipGetLastError(), "k_u_identity");
     }
     mm_upload_mark("order", s);
-    UP_TRY(u->join_weights(), "Unable to transfer graph weights to device");
-    mm_upload_mark("weights copy (overlapped)", nullptr);
-    g->push_ok_struct = 0;
-    g->push_ok_weights = 0;
-    if (sorted && nn) {
-        unsigned long long *counts = nullptr;
-        if ((rc = tmp_alloc(u, &counts, 2, "symmetry counts")) != MM_OK) return rc;
-        if (want_rev && g->nnz) {
-            if ((rc = tmp_alloc(u, &u->rev, (size_t)g->nnz, "reverse positions")) != MM_OK) return rc;
-            UP_TRY(hipMemsetAsync(u->rev, 0xFF, sizeof(int32_t) * (size_t)g->nnz, s), "reverse positions");  // -1: none
-        }
-        UP_TRY(hipMemsetAsync(counts, 0, 2 * sizeof(unsigned long long), s), "symmetry counts");
-        UP_TRY(hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), s), "symmetry flags");
-        hipLaunchKernelGGL(k_u_sym, dim3(grid_for((int64_t)nn * 8)), dim3(UB), 0, s, u->rg, u->nb, u->w, g->n, u->rev,
-                           counts, flags);
-        if (u->nlong > 0)
-            hipLaunchKernelGGL(k_u_sym_long, dim3(std::min(u->nlong, 4096)), dim3(UB), 0, s, u->rg, u->nb, u->w, u->longs,
-                               u->nlong, u->rev, counts, flags);
-        UP_TRY(hipGetLastError(), "k_u_sym");
-        unsigned long long h_counts[2];
-        int32_t h_flags[2];
-        UP_TRY(hipMemcpyAsync(h_counts, counts, sizeof(h_counts), hipMemcpyDeviceToHost, s), "symmetry counts");
-        UP_TRY(copy_on(s, h_flags, flags, sizeof(h_flags), hipMemcpyDeviceToHost), "symmetry flags");
-        const bool sym = !h_flags[0] && h_counts[0] == h_counts[1];
-        g->push_ok_struct = sym;
-        g->push_ok_weights = sym && !h_flags[1];
-        tmp_free(u, counts);
-        if (!g->push_ok_weights && u->rev) {
-            tmp_free(u, u->rev);
-            u->rev = nullptr;
-        }
-    }
+    u->want_rev = want_rev;
     tmp_free(u, flags);
-    mm_upload_mark("symmetry test", s);
     *out = u;
     u = nullptr;  // (the guard deletes nothing now)
     return MM_OK;
@@ -1471,6 +1553,7 @@ int32_t mm_upload_finish(mm_graph *g, UploadState *u, bool align_rows, bool head
         UP_TRY(hipGetLastError(), "k_u_wcsr");
     }
     mm_upload_mark("ids and rows", s);
+    if ((rc = symmetry_test(u)) != MM_OK) return rc;
     int32_t max_loc = n > 0 ? n - 1 : 0;
     if (u->voff) {
         max_loc = 0;
@@ -1492,7 +1575,7 @@ int32_t mm_upload_finish(mm_graph *g, UploadState *u, bool align_rows, bool head
         if ((rc = tmp_alloc(u, &lists, 3 * nn, "row classes")) != MM_OK || (rc = tmp_alloc(u, &counts, 3, "row classes")) != MM_OK)
             return rc;
         UP_TRY(hipMemsetAsync(counts, 0, 3 * sizeof(int32_t), s), "row classes");
-        hipLaunchKernelGGL(k_u_classify, dim3(grid_for(n)), dim3(UB), 0, s, g->d_ranges, n, lists, n, counts);
+        hipLaunchKernelGGL(k_u_classify, dim3(grid_red(u, n)), dim3(UB), 0, s, g->d_ranges, n, lists, n, counts);
         UP_TRY(hipGetLastError(), "k_u_classify");
         int32_t h_counts[3];
         UP_TRY(hipMemcpyAsync(h_counts, counts, sizeof(h_counts), hipMemcpyDeviceToHost, s), "row classes");
@@ -1587,7 +1670,7 @@ int32_t mm_upload_finish(mm_graph *g, UploadState *u, bool align_rows, bool head
             (rc = tmp_alloc(u, &cnt, 8, "vertex order")) != MM_OK)
             return rc;
         UP_TRY(hipMemsetAsync(cnt, 0, 8 * sizeof(int32_t), s), "vertex order");
-        hipLaunchKernelGGL(k_u_eighths, dim3(grid_for(n)), dim3(UB), 0, s, inv, n, ka, va, cnt);
+        hipLaunchKernelGGL(k_u_eighths, dim3(grid_red(u, n)), dim3(UB), 0, s, inv, n, ka, va, cnt);
         UP_TRY(hipGetLastError(), "k_u_eighths");
         if ((rc = sort_pairs(u, ka, kb, va, vb, n, 3)) != MM_OK) return rc;
         if ((rc = mm_graph_alloc(g, (void **)&g->d_unperm, sizeof(int2) * nn, "vertex order")) != MM_OK ||
