@@ -388,3 +388,43 @@ def test_push_eligibility_of_crafted_csrs():
             res = g.match(kind=MM_WEIGHTED, maximal=True, seeds=SEEDS, respond=respond)
             exp = oracle.match(c, True, True, SEEDS)[0]
             np.testing.assert_array_equal(res.match, exp, err_msg=str(k))
+
+
+@pytest.mark.parametrize("keep", [False, True])
+def test_dense_rows_exercise_the_construction_paths(keep):
+    """Rows of 100-400 entries: the construction's frontier buffers overflow
+    into the global list, rows fall into every length class (rank sorts of up
+    to 16 and 64 entries, radix-sorted longer rows, rows past the 256-entry
+    block threshold of the checks and the symmetry test)."""
+    rng = np.random.default_rng(29)
+    n = 1 << 17
+    q = n // 4
+    # ids [0, q): rows of about 8; [q, 2q): about 40; [2q, n): about 240 (a
+    # search level there reaches more new vertices per block than its LDS
+    # buffer holds); a few cross edges and 16 rows of about 600
+    parts = [(0, q, 4 * q), (q, 2 * q, 20 * q), (2 * q, n, 240 * q)]
+    a = np.concatenate([rng.integers(lo, hi, m) for lo, hi, m in parts] + [rng.integers(0, q, 4000)])
+    b = np.concatenate([rng.integers(lo, hi, m) for lo, hi, m in parts] + [rng.integers(q, n, 4000)])
+    hubs = rng.integers(0, n, 16)
+    a = np.concatenate([a, np.repeat(hubs, 350)])
+    b = np.concatenate([b, rng.integers(0, n, 16 * 350)])
+    keep_e = a != b
+    lo_, hi_ = np.minimum(a[keep_e], b[keep_e]), np.maximum(a[keep_e], b[keep_e])
+    key = np.unique(lo_.astype(np.int64) * n + hi_)
+    e = np.stack([key // n, key % n], 1)
+    w = ((rng.integers(0, 1 << 24, len(e)) + 1) * 2.0 ** -24).astype(np.float32)
+    csr = CSR.from_edges(n, e, w)
+    deg = csr.ranges[:, 1] - csr.ranges[:, 0]
+    say("dense graph", n, "nnz", csr.nnz, "max degree", int(deg.max()), "keep", keep)
+    assert deg.max() > 256 and (deg <= 64).any() and ((deg > 64) & (deg <= 256)).any()
+    g = Graph(csr, keep_order=keep)
+    assert g.info()["push_ok"]
+    for maximal in (False, True):
+        res = check_against_oracle(csr, g, True, maximal)
+        ok, under, multi, invalid, matchable = oracle.test_matching(csr, res.match)
+        wt, size = oracle.get_weight(csr, res.match)
+        chk = g.check()
+        assert (chk["under"], chk["multi"], chk["invalid"], chk["matchable"], chk["size"]) == \
+            (under, multi, invalid, matchable, size)
+        assert chk["weight"] == wt and chk["valid"] == int(ok)
+    g.close()
