@@ -425,11 +425,27 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
     const unsigned long long gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (gw * G));
     const int step = (int)(blockDim.x >> 6) * GPW;
     // list entries are read at clamped indices (always in bounds); validity
-    // is decided by the unclamped index
-    auto ent_at = [&](int32_t i) { return load_ent(list + min(i, count - 1)); };
+    // is decided by the unclamped index.  With 16 or fewer rows per
+    // wavefront the step's entries are one coalesced dword load (lane q
+    // reads word q & 3 of entry q >> 2) and every group takes its entry's
+    // words by shuffles, instead of every lane loading its group's 16 bytes.
+    constexpr bool WORDS = GPW <= 16;
+    auto ent_at = [&](int32_t i0) -> int4 {
+        if (WORDS) {
+            const int q = lane & (4 * GPW - 1);
+            const int32_t w = __builtin_nontemporal_load(
+                reinterpret_cast<const int32_t *>(list + min(i0 + (q >> 2), count - 1)) + (q & 3));
+            return make_int4(w, 0, 0, 0);
+        }
+        return load_ent(list + min(i0 + gw, count - 1));
+    };
+    auto ent_of = [&](const int4 &x) -> int4 {
+        if (!WORDS) return x;
+        return make_int4(__shfl(x.x, gw * 4), __shfl(x.x, gw * 4 + 1), __shfl(x.x, gw * 4 + 2), __shfl(x.x, gw * 4 + 3));
+    };
     int first = (threadIdx.x >> 6) * GPW;
-    int4 cur = ent_at(first + gw);
-    int4 nxt = ent_at(first + step + gw);
+    int4 cur = ent_of(ent_at(first));
+    int4 nxt = ent_at(first + step);
     int2 praw = first_raw<A8>(d, cur, gl);  // first step of cur's row
     for (; first < count; first += step) {
         const bool valid = first + gw < count;
@@ -448,8 +464,9 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
         const int32_t q0 = RESPOND ? d.req[g0] : 0;
         // keep the scheduler from hoisting the prefetches above the gathers
         __builtin_amdgcn_sched_barrier(0);
-        const int4 nn = ent_at(first + 2 * step + gw);
-        const int2 nraw = first_raw<A8>(d, nxt, gl);
+        const int4 nn = ent_at(first + 2 * step);
+        const int4 nxe = ent_of(nxt);
+        const int2 nraw = first_raw<A8>(d, nxe, gl);
         // step 0
         int32_t win_u = -1;
         float win_w = 1.0f;
@@ -478,7 +495,7 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
             if (!deferred) first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp);
             nnz += (unsigned long long)(rg.y - rg.x);
         }
-        cur = nxt;
+        cur = nxe;
         nxt = nn;
         praw = nraw;
     }
