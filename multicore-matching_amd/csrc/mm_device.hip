@@ -723,8 +723,67 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per
     // row ranges serve the hub test, the round's lists and the degree sums
     // only: without lists, hubs or sums they are not read.
     const bool need_rows = !d.listless1 || !d.no_hubs || d.count_nnz;
+    // Without lists or hubs, a lane takes 4 consecutive ids: one 16-byte
+    // load of their original ids, one 4-byte store of their states, one
+    // 16-byte store (32 with 64-bit keys) of their empty keys.  The address
+    // unit's work grows with the lanes a memory instruction serves, so 4 ids
+    // per lane cost a fraction of 4 single-id accesses.
+    // (the degree sums of a profiled run read the ranges as two more 16-byte
+    // loads per lane)
+    const bool quads = d.listless1 && d.no_hubs && (lo & 3) == 0;
+    if (quads) {
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        int32_t nblue = 0, nred = 0;
+        const int64_t nq = (hi - lo) >> 2;
+        for (int64_t k = threadIdx.x; k < nq; k += blockDim.x) {
+            const int64_t v0 = lo + 4 * k;
+            const v4i l4 = *reinterpret_cast<const v4i *>(d.loc + v0);
+            if (d.count_nnz) {
+                const v4i *rp = reinterpret_cast<const v4i *>(d.ranges + v0);
+                const v4i r01 = rp[0], r23 = rp[1];
+                deg_sum += (unsigned long long)((r01.y - r01.x) + (r01.w - r01.z) + (r23.y - r23.x) + (r23.w - r23.z));
+            }
+            const int32_t lv[4] = {l4.x, l4.y, l4.z, l4.w};
+            uint32_t packed = 0u;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int c = colour_of((uint32_t)lv[q], seed, d.barrier);
+                packed |= (uint32_t)c << (8 * q);
+                nblue += c == ST_BLUE;
+                nred += c == ST_RED;
+            }
+            *reinterpret_cast<uint32_t *>(d.st + v0) = packed;
+            if (d.vround) *reinterpret_cast<uint32_t *>(d.vround + v0) = 0xFFFFFFFFu;
+            if (PUSH) {
+                if (d.key32) {
+                    *reinterpret_cast<v4i *>(reinterpret_cast<uint32_t *>(resp_slot(d, parity)) + v0) =
+                        v4i{-1, -1, -1, -1};
+                } else {
+                    v4i *kp = reinterpret_cast<v4i *>(resp_slot(d, parity) + v0);
+                    kp[0] = v4i{0, 0, 0, 0};
+                    kp[1] = v4i{0, 0, 0, 0};
+                }
+            }
+        }
+        // the last (hi - lo) % 4 ids, one per thread
+        for (int64_t v = lo + 4 * nq + threadIdx.x; v < hi; v += blockDim.x) {
+            if (d.count_nnz) deg_sum += (unsigned long long)(d.ranges[v].y - d.ranges[v].x);
+            const int c = colour_of((uint32_t)d.loc[v], seed, d.barrier);
+            d.st[v] = (uint8_t)c;
+            if (d.vround) d.vround[v] = 255;
+            nblue += c == ST_BLUE;
+            nred += c == ST_RED;
+            if (PUSH) clear_key(d, resp_slot(d, parity), (int32_t)v);
+        }
+        nblue = wave_sum(nblue);
+        nred = wave_sum(nred);
+        if ((threadIdx.x & 63) == 0) {
+            if (nblue) atomicAdd(s_cnt + 0, nblue);
+            if (nred) atomicAdd(s_cnt + 1, nred);
+        }
+    }
     constexpr int IU = 2;
-    for (int64_t base = lo + (threadIdx.x & ~63); base < hi; base += (int64_t)blockDim.x * IU) {
+    for (int64_t base = lo + (threadIdx.x & ~63); !quads && base < hi; base += (int64_t)blockDim.x * IU) {
         int32_t vl[IU];
         int2 rg[IU];
 #pragma unroll
