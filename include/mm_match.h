@@ -106,7 +106,9 @@ typedef struct mm_graph_info {
                                  edge in the red vertex's sorted row, then local id: needs
                                  max_degree < 2^(32 - bits of the largest local id));
                                  64: (weight, local id) keys; 0: no PUSH */
-    int32_t reserved;
+    int32_t iso_settled;      /* 1: at least 1% of the vertices have no neighbour, and the first round's
+                                 select settles those (their colours in every round follow from the
+                                 seeds alone); the work lists then carry only the others */
 } mm_graph_info;
 
 /* Last error message of the calling thread ("" when none). */

@@ -107,6 +107,11 @@ struct Dev {
     const int32_t *loc;        // graph-local ORIGINAL id of each (internally renumbered) vertex:
                                // the hash input, the proposal tie-break id and the match value base
     uint8_t *vround;           // batched: round in which the vertex stopped being active (255: never)
+    int32_t *iso;              // vertices without neighbours settled by the round-1 select (null: they
+                               // go through the rounds): per round r and bucket b, iso[(2r + c) nb + b]
+                               // of them were blue (c = 0) / red (c = 1) in round r; iso[2 ROUND_SLOTS nb
+                               // + b] = the last such round of bucket b
+    int32_t R_run;             // rounds this run may execute
 };
 
 // Work-list entry: the vertex, its row range and its original id travel
@@ -135,6 +140,10 @@ __device__ __forceinline__ int bucket_of_block(const Dev &d)
 }
 
 __device__ __forceinline__ uint32_t select_hash(uint32_t i, uint32_t seed) { return mm_select_hash(i, seed); }
+
+// Result index base of internal vertex v: its graph's first caller id
+// (batches; 0 for a single graph).  v's value goes to out[out_base + loc[v]].
+__device__ __forceinline__ int32_t out_base(const Dev &d, int32_t v) { return d.vbase ? d.vbase[v] : 0; }
 
 __device__ __forceinline__ int colour_of(uint32_t local_id, uint32_t seed, uint32_t barrier)
 {
@@ -666,6 +675,19 @@ __device__ __forceinline__ void bucket_append(bool pred, int4 value, int4 *bucke
     if (pred) bucket[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
+// bucket_append of an 8-byte entry.
+__device__ __forceinline__ void bucket_append2(bool pred, int2 value, int2 *bucket, int32_t *s_count)
+{
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(s_count, (int)__popcll(mask));
+    base = __shfl(base, leader);
+    if (pred) bucket[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = value;
+}
+
 // bucket_append without the entry: counts the lanes with pred set.
 __device__ __forceinline__ void bucket_count(bool pred, int32_t *s_count)
 {
@@ -835,6 +857,182 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        ctrb_ptr(d, 1, 0)[b] = s_cnt[0];
+        ctrb_ptr(d, 1, 1)[b] = s_cnt[1];
+    }
+}
+
+// Vertices without neighbours (select of round 1, graphs with many of them).
+// Such a vertex is never proposed to and never proposes: in every round it is
+// red and stays, or blue and dies (its request is n + 1, src/matchgpu.cu:
+// 320-325).  Its whole run depends only on its id and the round seeds, so the
+// round-1 select settles it at once instead of carrying it through every
+// round's lists: it is marked dead for the kernels that scan by id, its final
+// value is written (2 in the round it is first blue; its colour, red, if it
+// is red to the last round), and the rounds' counters get its share later
+// (k_reduce_counts), so round counts and statistics are unchanged.
+//
+// The red ones are queued, per bucket, in the bucket's segments of the next
+// round's work lists (round 2's lists are written only by round 1's match),
+// as {id, original id} pairs; after the bucket's scan the block takes the
+// queue through rounds 2, 3, ... together, each round hashing only the ones
+// still red and compacting them into the other segment.
+__device__ void iso_rounds(const Dev &d, int2 *qa, int2 *qb, int32_t *s_q, int32_t *s_iso, int32_t *s_last)
+{
+    const int lane = threadIdx.x & 63;
+    for (int r = 2; r <= d.R_run; ++r) {
+        __syncthreads();
+        const int32_t qn = s_q[0];
+        if (qn == 0) break;
+        if (threadIdx.x == 0) {
+            s_q[1] = 0;
+            if (r > *s_last) *s_last = r;
+        }
+        __syncthreads();
+        const uint32_t seed = d.seeds[r - 1];
+        int32_t nb = 0, nr = 0;
+        // (every lane of a wavefront reaches the append: the loop bound is wave-uniform)
+        for (int32_t i0 = threadIdx.x & ~63; i0 < qn; i0 += blockDim.x) {
+            const int32_t i = i0 + lane;
+            bool keep = false;
+            int2 e = make_int2(0, 0);
+            if (i < qn) {
+                e = qa[i];
+                if (colour_of((uint32_t)e.y, seed, d.barrier) == ST_BLUE) {
+                    d.out[out_base(d, e.x) + e.y] = 2;
+                    if (d.vround) d.vround[e.x] = (uint8_t)r;
+                    ++nb;
+                } else {
+                    keep = true;
+                    ++nr;
+                }
+            }
+            bucket_append2(keep, e, qb, s_q + 1);
+        }
+        nb = wave_sum(nb);
+        nr = wave_sum(nr);
+        if (lane == 0) {
+            if (nb) atomicAdd(s_iso + 2 * r, nb);
+            if (nr) atomicAdd(s_iso + 2 * r + 1, nr);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) s_q[0] = s_q[1];
+        int2 *t = qa;
+        qa = qb;
+        qb = t;
+    }
+    // red to the last round: the run ends with them active and red
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < s_q[0]; i += blockDim.x) {
+        const int2 e = qa[i];
+        d.out[out_base(d, e.x) + e.y] = ST_RED;
+    }
+}
+
+// k_init_select's general path (row ranges read, lists or hubs) for graphs
+// whose vertices without neighbours are settled here (Dev::iso).
+template <bool PUSH>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per_eu(8))) void k_init_select_iso(Dev d)
+{
+    constexpr int IU = 2;
+    __shared__ int32_t s_cnt[2], s_q[2], s_last;
+    __shared__ int32_t s_iso[2 * ROUND_SLOTS];
+    const int parity = 1, b = bucket_of_block(d);
+    int2 *qa = reinterpret_cast<int2 *>(list_ptr(d, 0, L_BLUE_LO) + (size_t)b * d.bcap);
+    int2 *qb = reinterpret_cast<int2 *>(list_ptr(d, 0, L_RED_LO) + (size_t)b * d.bcap);
+    if (threadIdx.x < 2) {
+        s_cnt[threadIdx.x] = 0;
+        s_q[threadIdx.x] = 0;
+    }
+    if (threadIdx.x == 0) s_last = 0;
+    for (int i = threadIdx.x; i < 2 * ROUND_SLOTS; i += blockDim.x) s_iso[i] = 0;
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < ROUND_SLOTS * CTR_STRIDE; i += blockDim.x) d.ctr_next[i] = 0;
+        for (int i = threadIdx.x; i < ROUND_SLOTS * 2; i += blockDim.x) d.hub64_next[i] = 0ull;
+    }
+    __syncthreads();
+    const uint32_t seed = d.seeds[0];
+    const int64_t lo = (int64_t)b * d.bcap, hi = min((int64_t)d.n, lo + d.bcap);
+    int4 *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
+    int4 *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
+    unsigned long long deg_sum = 0, hub_deg_sum = 0;
+    int32_t iso_b = 0, iso_r = 0;
+    for (int64_t base = lo + (threadIdx.x & ~63); base < hi; base += (int64_t)blockDim.x * IU) {
+        int32_t vl[IU];
+        int2 rg[IU];
+#pragma unroll
+        for (int q = 0; q < IU; ++q) {
+            const int64_t v = base + (int64_t)q * blockDim.x + (threadIdx.x & 63);
+            vl[q] = 0;
+            rg[q] = make_int2(0, 0);
+            if (v < hi) {
+                vl[q] = d.loc[v];
+                rg[q] = d.ranges[v];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < IU; ++q) {
+            const int32_t v = (int32_t)(base + (int64_t)q * blockDim.x + (threadIdx.x & 63));
+            int cls = -1;
+            bool queue = false;
+            const int4 ent = make_int4(v, rg[q].x, rg[q].y, vl[q]);
+            if (v < hi) {
+                const int c = colour_of((uint32_t)vl[q], seed, d.barrier);
+                if (d.vround) d.vround[v] = 255;
+                const int32_t deg = rg[q].y - rg[q].x;
+                if (deg == 0) {
+                    d.st[v] = ST_DEAD;
+                    if (c == ST_BLUE) {
+                        d.out[out_base(d, v) + vl[q]] = 2;
+                        if (d.vround) d.vround[v] = 1;
+                        ++iso_b;
+                    } else {
+                        queue = true;
+                        ++iso_r;
+                    }
+                } else {
+                    d.st[v] = (uint8_t)c;
+                    const int hub = deg > d.hub_deg ? 1 : 0;
+                    cls = 2 * c + hub;
+                    if (hub) hub_deg_sum += (unsigned long long)deg;
+                    else deg_sum += (unsigned long long)deg;
+                }
+                if (PUSH) clear_key(d, resp_slot(d, parity), v);
+            }
+            bucket_append2(queue, make_int2(v, vl[q]), qa, s_q);
+            if (d.listless1) {
+                bucket_count(cls == L_BLUE_LO, s_cnt + 0);
+                bucket_count(cls == L_RED_LO, s_cnt + 1);
+            } else {
+                bucket_append(cls == L_BLUE_LO, ent, blue, s_cnt + 0);
+                bucket_append(cls == L_RED_LO, ent, red, s_cnt + 1);
+            }
+            wave_append(cls == L_BLUE_HI, ent, list_ptr(d, parity, L_BLUE_HI), d.ctr + 1 * CTR_STRIDE + L_BLUE_HI);
+            wave_append(cls == L_RED_HI, ent, list_ptr(d, parity, L_RED_HI), d.ctr + 1 * CTR_STRIDE + L_RED_HI);
+        }
+    }
+    iso_rounds(d, qa, qb, s_q, s_iso, &s_last);
+    iso_b = wave_sum(iso_b);
+    iso_r = wave_sum(iso_r);
+    if ((threadIdx.x & 63) == 0) {
+        if (iso_b) atomicAdd(s_iso + 2, iso_b);
+        if (iso_r) atomicAdd(s_iso + 3, iso_r);
+    }
+    if (d.count_nnz) {
+        deg_sum = block_sum(deg_sum);
+        hub_deg_sum = block_sum(hub_deg_sum);
+        if (threadIdx.x == 0) {
+            ctrb64_ptr(d, 1, 0)[b] = deg_sum;
+            if (hub_deg_sum) atomicAdd(d.hub64 + 1 * 2 + 0, hub_deg_sum);
+        }
+    }
+    __syncthreads();
+    // this run's isolated counts of the bucket, round by round, and its last round
+    const int32_t last = s_iso[2] + s_iso[3] > 0 ? max(s_last, 1) : 0;
+    for (int i = threadIdx.x; i < 2 * (last + 1); i += blockDim.x)
+        if (i >= 2) d.iso[(size_t)i * d.nb + b] = s_iso[i];
+    if (threadIdx.x == 0) {
+        d.iso[(size_t)2 * ROUND_SLOTS * d.nb + b] = last;
         ctrb_ptr(d, 1, 0)[b] = s_cnt[0];
         ctrb_ptr(d, 1, 1)[b] = s_cnt[1];
     }
@@ -1081,7 +1279,6 @@ __global__ void k_respond(Dev d, int r, int hub_blocks)
 // (a 4-byte scattered store per vertex over the whole run, instead of a
 // separate permutation pass at its end).  out index of a vertex with
 // graph-local id loc in the graph of internal vertex v: vbase[v] + loc.
-__device__ __forceinline__ int32_t out_base(const Dev &d, int32_t v) { return d.vbase ? d.vbase[v] : 0; }
 
 // partner_of's value for a blue vertex of a PUSH round that won its proposal:
 // its result is written by the red end, which knows both ids.
@@ -1220,6 +1417,8 @@ void k_match_select(Dev d, int r, int do_select)
                     const int2 rg = d.ranges[vv];
                     e[q] = make_int4(vv, rg.x, rg.y, d.loc[vv]);
                     col[q] = d.st[vv];
+                    // (settled by the select: a vertex without neighbours)
+                    if (col[q] >= ST_DEAD) col[q] = -1;
                 }
             } else if (t < total) {
                 col[q] = t < c0 ? ST_BLUE : ST_RED;
@@ -1321,6 +1520,16 @@ __global__ void k_reduce_counts(Dev d, int r0, int with_nnz)
             z[1] += ctrb64_ptr(d, r, 1)[b];
         }
     }
+    // vertices without neighbours settled by the round-1 select (Dev::iso)
+    int32_t ib = 0, ir = 0;
+    if (d.iso)
+        for (int b = threadIdx.x; b < d.nb; b += blockDim.x)
+            if (r <= d.iso[(size_t)2 * ROUND_SLOTS * d.nb + b]) {
+                ib += d.iso[(size_t)(2 * r) * d.nb + b];
+                ir += d.iso[(size_t)(2 * r + 1) * d.nb + b];
+            }
+    ib = block_sum(ib);
+    ir = block_sum(ir);
 #pragma unroll
     for (int j = 0; j < 4; ++j) a[j] = block_sum(a[j]);
     if (with_nnz) {
@@ -1329,10 +1538,10 @@ __global__ void k_reduce_counts(Dev d, int r0, int with_nnz)
     }
     if (threadIdx.x == 0) {
         int32_t *c = d.ctr + (size_t)r * CTR_STRIDE;
-        c[L_BLUE_LO] = a[0];
-        c[L_RED_LO] = a[1];
+        c[L_BLUE_LO] = a[0] + ib;
+        c[L_RED_LO] = a[1] + ir;
         c[4] = a[2];
-        c[5] = a[3];
+        c[5] = a[3] + ib;
         if (with_nnz) {
             d.ctr64[(size_t)r * CTR64_STRIDE + 0] = z[0] + d.hub64[(size_t)r * 2 + 0];
             d.ctr64[(size_t)r * CTR64_STRIDE + 1] = z[1] + d.hub64[(size_t)r * 2 + 1];
@@ -1495,7 +1704,7 @@ void release(mm_graph *g)
     if (g->stream) hipStreamSynchronize(g->stream);
     void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_adjp,   g->d_head,   g->d_headc,  g->d_match, g->d_req,   g->d_st,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
-                    g->d_hub64,  g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
+                    g->d_hub64,  g->d_iso,   g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
                     g->d_unperm, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_lcnt, g->d_llist, g->d_params, g->d_vbase, g->d_chk, g->d_chk_keys, g->d_wcsr,
                     g->d_lsamp};
     for (void *p : ptrs)
@@ -1575,6 +1784,12 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     if ((rc = dev_alloc(g, &g->d_ctrb64, (size_t)ROUND_SLOTS * 2 * g->nb_max, "bucket counters")) != MM_OK)
         return fail(rc);
     if ((rc = dev_alloc(g, &g->d_hub64, (size_t)2 * ROUND_SLOTS * 2, "counters")) != MM_OK) return fail(rc);
+    int64_t with_nbrs = 0;
+    for (int k = 0; k < 33; ++k) with_nbrs += g->deg_hist[k];
+    g->n_isolated = (int64_t)n - with_nbrs;
+    if (g->n_isolated * 100 >= (int64_t)n && g->n_isolated > 0 &&
+        (rc = dev_alloc(g, &g->d_iso, ((size_t)ROUND_SLOTS * 2 + 1) * g->nb_max, "isolated counters")) != MM_OK)
+        return fail(rc);
     if (hipMemset(g->d_ctr, 0, sizeof(int32_t) * 2 * ROUND_SLOTS * CTR_STRIDE) != hipSuccess ||
         hipMemset(g->d_hub64, 0, sizeof(unsigned long long) * 2 * ROUND_SLOTS * 2) != hipSuccess)
         return fail(mm::set_error(MM_ERR_HIP, "Unable to clear counters"));
@@ -1827,6 +2042,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.seeds = g->d_seeds;
     d.loc = g->d_loc;
     d.vround = g->d_vround;
+    d.iso = g->d_iso;
 
     const bool prof = o->profile != 0;
     const bool use_graph = !prof && o->graph_mode == 0;
@@ -1862,6 +2078,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
 
     // A maximal run may stop early; it can only go as far as the seeds given.
     const int R_run = maximal ? std::min(R_max, o->n_seeds) : R_max;
+    d.R_run = R_run;
     // Fixed runs are one launch sequence; maximal runs check the round
     // counters after every chunk of rounds (rounds past convergence are no-ops).
     // The first chunk is 4 rounds; later ones last as long as the decay of the
@@ -1877,7 +2094,9 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     // maximal runs also total the counters of rounds [r0, r1 + 1].
     auto enqueue = [&](int r0, int r1) -> int32_t {
         if (r0 == 1) {
-            if (c.PUSH) MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select<true>, dim3(c.nb), dim3(block), 0, g->stream, d));
+            if (d.iso && c.PUSH) MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select_iso<true>, dim3(c.nb), dim3(block), 0, g->stream, d));
+            else if (d.iso) MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select_iso<false>, dim3(c.nb), dim3(block), 0, g->stream, d));
+            else if (c.PUSH) MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select<true>, dim3(c.nb), dim3(block), 0, g->stream, d));
             else MM_LAUNCH(K_INIT, hipLaunchKernelGGL(k_init_select<false>, dim3(c.nb), dim3(block), 0, g->stream, d));
             if (!use_graph) flip.init_submitted = true;
         }
@@ -2216,7 +2435,7 @@ int32_t mm_graph_get_info(const mm_graph *g, mm_graph_info *info)
     info->push_ok = g->weighted ? g->push_ok_weights : g->push_ok_struct;
     info->device_bytes = g->device_bytes;
     info->key_bits = !info->push_ok ? 0 : (g->d_adjp ? 32 : 64);
-    info->reserved = 0;
+    info->iso_settled = g->d_iso ? 1 : 0;
     return MM_OK;
 }
 

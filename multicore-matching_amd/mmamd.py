@@ -97,7 +97,7 @@ class _Info(ctypes.Structure):
         ("push_ok", ctypes.c_int32),
         ("device_bytes", ctypes.c_int64),
         ("key_bits", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("iso_settled", ctypes.c_int32),
     ]
 
 
