@@ -112,6 +112,10 @@ struct Dev {
                                // of them were blue (c = 0) / red (c = 1) in round r; iso[2 ROUND_SLOTS nb
                                // + b] = the last such round of bucket b
     int32_t R_run;             // rounds this run may execute
+    int32_t n1, bcap1;         // the id range [0, n1) that round 1's request and match scan (without
+                               // lists), bcap1 ids per bucket: all ids, or those before the isolated
+                               // vertices the select settles (Dev::iso, a single graph that keeps
+                               // them last)
 };
 
 // Work-list entry: the vertex, its row range and its original id travel
@@ -1055,8 +1059,8 @@ __global__ void k_request(Dev d, int r, int hub_blocks)
     }
     const int b = bucket_of_block(d);
     if (HEAD)
-        head_scan_range<PUSH, 8, HC>(d, (int64_t)b * d.bcap, min((int64_t)d.n, (int64_t)(b + 1) * d.bcap), resp,
-                                     nnz);
+        head_scan_range<PUSH, 8, HC>(d, (int64_t)b * d.bcap1, min((int64_t)d.n1, (int64_t)(b + 1) * d.bcap1),
+                                     resp, nnz);
     else
         first_scan_bucket<W, A8, PUSH, false, G>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap,
                                                  ctrb_ptr(d, r, 0)[b], resp, nnz);
@@ -1386,9 +1390,9 @@ void k_match_select(Dev d, int r, int do_select)
     __syncthreads();
     // LL (round 1 without lists): the items are the bucket's ids, every one
     // active; rows above the hub degree are left to the hub part below.
-    const int64_t id0 = (int64_t)b * d.bcap;
+    const int64_t id0 = (int64_t)b * (LL ? d.bcap1 : d.bcap);
     const int32_t c0 = LL ? 0 : ctrb_ptr(d, r, 0)[b], c1 = LL ? 0 : ctrb_ptr(d, r, 1)[b];
-    const int32_t total = LL ? (int32_t)max((int64_t)0, min((int64_t)d.n, id0 + d.bcap) - id0) : c0 + c1;
+    const int32_t total = LL ? (int32_t)max((int64_t)0, min((int64_t)d.n1, id0 + d.bcap1) - id0) : c0 + c1;
     const int4 *blue = list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap;
     const int4 *red = list_ptr(d, parity, L_RED_LO) + (size_t)b * d.bcap;
     int4 *nblue = list_ptr(d, nparity, L_BLUE_LO) + (size_t)b * d.bcap;
@@ -2043,6 +2047,9 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.loc = g->d_loc;
     d.vround = g->d_vround;
     d.iso = g->d_iso;
+    // round 1 scans by id only the ids before the isolated vertices the select settles
+    d.n1 = g->d_iso && g->iso_lo < g->n ? g->iso_lo : g->n;
+    d.bcap1 = (int32_t)(((int64_t)d.n1 + c.nb - 1) / c.nb);
 
     const bool prof = o->profile != 0;
     const bool use_graph = !prof && o->graph_mode == 0;

@@ -56,6 +56,8 @@ struct mm_graph {
     unsigned long long *d_hub64 = nullptr;
     int32_t nb_max = 1;       // bucket slots allocated per counter row
     int64_t n_isolated = 0;   // vertices without neighbours
+    int32_t iso_lo = INT32_MAX; // a single relabelled graph with many of them: the first internal id of
+                              // those, which the internal order puts after all others
     int32_t *d_iso = nullptr; // graphs with many of them: per round slot and bucket, the isolated vertices
                               // blue and red in that round ([ROUND_SLOTS][2][nb_max]), then per bucket the
                               // last round with one ([nb_max])

@@ -544,14 +544,19 @@ __global__ void k_u_scatter_rank(const int32_t *seed, const int32_t *rank, int32
 }
 
 // Per graph: first position, length, isolated vertices, where its isolated
-// and its other vertices start in the sorted keys.
+// and its other vertices start in the sorted keys.  tail: each graph's
+// isolated vertices follow its other ones (graphs where they are many: the
+// first round's select settles them, and the first round's other kernels
+// then scan only the ids before them); else they are spread evenly.
 struct Segs {
     const int32_t *lo, *len, *k, *iso_at, *conn_at;
     int32_t count;
+    int32_t tail;
 };
 
 // order[p] for every position: position q of a graph of len positions and k
-// isolated vertices holds an isolated one when floor((q+1)k/len) steps up.
+// isolated vertices holds an isolated one when floor((q+1)k/len) steps up
+// (spread), or when q >= len - k (tail).
 __global__ void k_u_merge(const unsigned long long *sorted, Segs s, int32_t n, int32_t *order, int32_t *inv)
 {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
@@ -562,8 +567,13 @@ __global__ void k_u_merge(const unsigned long long *sorted, Segs s, int32_t n, i
             else hi = mid - 1;
         }
         const int64_t q = p - s.lo[lo], len = s.len[lo], k = s.k[lo];
-        const int64_t before = (q * k) / len, upto = ((q + 1) * k) / len;
-        const int64_t at = upto > before ? s.iso_at[lo] + before : s.conn_at[lo] + (q - before);
+        int64_t at;
+        if (s.tail) {
+            at = q < len - k ? s.conn_at[lo] + q : s.iso_at[lo] + (q - (len - k));
+        } else {
+            const int64_t before = (q * k) / len, upto = ((q + 1) * k) / len;
+            at = upto > before ? s.iso_at[lo] + before : s.conn_at[lo] + (q - before);
+        }
         const int32_t v = (int32_t)(sorted[at] & 0xFFFFFFFFull);
         order[p] = v;
         inv[v] = (int32_t)p;
@@ -1273,6 +1283,9 @@ int32_t locality_order(UploadState *u, int32_t *order, int32_t *inv)
     sg.iso_at = d_seg + 3 * (size_t)ng;
     sg.conn_at = d_seg + 4 * (size_t)ng;
     sg.count = ng;
+    // (the same rule as the matcher's settling of isolated vertices: at least 1% of the vertices)
+    sg.tail = iso_total > 0 && iso_total * 100 >= (int64_t)n ? 1 : 0;
+    if (sg.tail && !u->voff) u->g->iso_lo = (int32_t)(n - iso_total);
     hipLaunchKernelGGL(k_u_merge, dim3(grid_for(n)), dim3(UB), 0, s, sorted, sg, n, order, inv);
     UP_TRY(hipGetLastError(), "k_u_merge");
     UP_TRY(hipStreamSynchronize(s), "locality order");
