@@ -971,7 +971,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per
             rg[q] = make_int2(0, 0);
             if (v < hi) {
                 vl[q] = d.loc[v];
-                rg[q] = d.ranges[v];
+                // (ids from n1 on have no neighbours: their row ranges are not needed)
+                if (v < d.n1) rg[q] = d.ranges[v];
             }
         }
 #pragma unroll
@@ -985,7 +986,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per
                 if (d.vround) d.vround[v] = 255;
                 const int32_t deg = rg[q].y - rg[q].x;
                 if (deg == 0) {
-                    d.st[v] = ST_DEAD;
+                    // (no kernel reads the state of an id from n1 on, nor the key of a
+                    // vertex without neighbours: nobody proposes to it)
+                    if (v < d.n1) d.st[v] = ST_DEAD;
                     if (c == ST_BLUE) {
                         d.out[out_base(d, v) + vl[q]] = 2;
                         if (d.vround) d.vround[v] = 1;
@@ -1000,8 +1003,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per
                     cls = 2 * c + hub;
                     if (hub) hub_deg_sum += (unsigned long long)deg;
                     else deg_sum += (unsigned long long)deg;
+                    if (PUSH) clear_key(d, resp_slot(d, parity), v);
                 }
-                if (PUSH) clear_key(d, resp_slot(d, parity), v);
             }
             bucket_append2(queue, make_int2(v, vl[q]), qa, s_q);
             if (d.listless1) {
@@ -1510,12 +1513,15 @@ void k_match_select(Dev d, int r, int do_select)
 }
 
 // Sums the per-bucket counters of rounds [r0, r0 + gridDim.x) into the
-// per-round totals (one block per round).  Assigns, so it may be re-run.
-__global__ void k_reduce_counts(Dev d, int r0, int with_nnz)
+// per-round totals (one block of up to 1024 threads per round; every thread
+// issues all its bucket loads before adding, and the eight sums share one
+// wavefront reduction and one LDS pass).  Assigns, so it may be re-run.
+__global__ __attribute__((amdgpu_flat_work_group_size(1, 1024))) void k_reduce_counts(Dev d, int r0, int with_nnz)
 {
     const int r = r0 + blockIdx.x;
-    int32_t a[4] = {0, 0, 0, 0};
+    int32_t a[6] = {0, 0, 0, 0, 0, 0};
     unsigned long long z[2] = {0ull, 0ull};
+    const int32_t *iso_last = d.iso ? d.iso + (size_t)2 * ROUND_SLOTS * d.nb : nullptr;
     for (int b = threadIdx.x; b < d.nb; b += blockDim.x) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) a[j] += ctrb_ptr(d, r, j)[b];
@@ -1523,29 +1529,45 @@ __global__ void k_reduce_counts(Dev d, int r0, int with_nnz)
             z[0] += ctrb64_ptr(d, r, 0)[b];
             z[1] += ctrb64_ptr(d, r, 1)[b];
         }
-    }
-    // vertices without neighbours settled by the round-1 select (Dev::iso)
-    int32_t ib = 0, ir = 0;
-    if (d.iso)
-        for (int b = threadIdx.x; b < d.nb; b += blockDim.x)
-            if (r <= d.iso[(size_t)2 * ROUND_SLOTS * d.nb + b]) {
-                ib += d.iso[(size_t)(2 * r) * d.nb + b];
-                ir += d.iso[(size_t)(2 * r + 1) * d.nb + b];
+        // vertices without neighbours settled by the round-1 select (Dev::iso);
+        // a bucket's slots past its last round hold stale values
+        if (iso_last) {
+            const int32_t last = iso_last[b];
+            const int32_t ib = d.iso[(size_t)(2 * r) * d.nb + b], ir = d.iso[(size_t)(2 * r + 1) * d.nb + b];
+            if (r <= last) {
+                a[4] += ib;
+                a[5] += ir;
             }
-    ib = block_sum(ib);
-    ir = block_sum(ir);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) a[j] = block_sum(a[j]);
-    if (with_nnz) {
-        z[0] = block_sum(z[0]);
-        z[1] = block_sum(z[1]);
+        }
     }
+    __shared__ int32_t s_a[16][6];
+    __shared__ unsigned long long s_z[16][2];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) a[j] = wave_sum(a[j]);
+    if (with_nnz) {
+        z[0] = wave_sum(z[0]);
+        z[1] = wave_sum(z[1]);
+    }
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int j = 0; j < 6; ++j) s_a[w][j] = a[j];
+        s_z[w][0] = z[0];
+        s_z[w][1] = z[1];
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        for (int x = 1; x < (int)((blockDim.x + 63) >> 6); ++x) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j) a[j] += s_a[x][j];
+            z[0] += s_z[x][0];
+            z[1] += s_z[x][1];
+        }
         int32_t *c = d.ctr + (size_t)r * CTR_STRIDE;
-        c[L_BLUE_LO] = a[0] + ib;
-        c[L_RED_LO] = a[1] + ir;
+        c[L_BLUE_LO] = a[0] + a[4];
+        c[L_RED_LO] = a[1] + a[5];
         c[4] = a[2];
-        c[5] = a[3] + ib;
+        c[5] = a[3] + a[4];
         if (with_nnz) {
             d.ctr64[(size_t)r * CTR64_STRIDE + 0] = z[0] + d.hub64[(size_t)r * 2 + 0];
             d.ctr64[(size_t)r * CTR64_STRIDE + 1] = z[1] + d.hub64[(size_t)r * 2 + 1];
@@ -2093,7 +2115,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     // counters back twice.
     int chunk = maximal ? 4 : R_run;
     auto enq_reduce = [&](int r0, int r1) -> int32_t {
-        MM_LAUNCH(K_CLEAR, hipLaunchKernelGGL(k_reduce_counts, dim3(r1 - r0 + 1), dim3(256), 0, g->stream, d, r0,
+        MM_LAUNCH(K_CLEAR, hipLaunchKernelGGL(k_reduce_counts, dim3(r1 - r0 + 1), dim3(1024), 0, g->stream, d, r0,
                                               d.count_nnz));
         return MM_OK;
     };
