@@ -79,6 +79,7 @@ struct Dev {
     const uint32_t *headc;// the same in 4 bytes per entry: invalid << 31 | id << head_pb | position
                           // (graphs with 32-bit keys whose ids and positions fit; null otherwise)
     int32_t head_pb;
+    const uint32_t *adjc; // every entry of the sorted rows packed as headc packs them (or null)
     int32_t *out;         // result in the caller's order, written as vertices settle
     const int32_t *vbase; // batches: first caller id of each vertex's graph (null: single graph)
     int32_t *req;
@@ -168,14 +169,27 @@ __device__ __forceinline__ void wave_append(bool pred, int4 value, int4 *list, i
     if (pred) list[base + (int)__popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
+// Row entry formats (template parameter A8): 0 ids only (d.nbrs, unweighted
+// rows), 1 {id, weight bits or key position} pairs (d.adj), 2 the pairs with
+// key positions packed in 4 bytes, invalid << 31 | id << head_pb | position
+// (d.adjc).
+__device__ __forceinline__ void unpack_entry(const Dev &d, uint32_t x, int32_t &u, bool &valid, float &w)
+{
+    valid = (x >> 31) == 0u;
+    u = (int32_t)((x >> d.head_pb) & ((1u << (31 - d.head_pb)) - 1u));
+    w = __int_as_float((int32_t)(x & ((1u << d.head_pb) - 1u)));  // (the key position)
+}
+
 // Neighbour id of row entry j; valid is false for a weight that can never
 // win (sorted weighted rows store those ids complemented).  Rows are streamed
 // once per round: non-temporal loads, so they do not evict the gathered
 // state arrays from L2.
-template <bool A8>
+template <int A8>
 __device__ __forceinline__ void load_id(const Dev &d, int32_t j, int32_t &u, bool &valid, float &w)
 {
-    if (A8) {
+    if (A8 == 2) {
+        unpack_entry(d, __builtin_nontemporal_load(d.adjc + j), u, valid, w);
+    } else if (A8) {
         typedef int v2i __attribute__((ext_vector_type(2)));
         const v2i e = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(d.adj + j));
         valid = e.x >= 0;
@@ -190,10 +204,11 @@ __device__ __forceinline__ void load_id(const Dev &d, int32_t j, int32_t &u, boo
 
 // Raw row entry j ({id, weight bits} weighted, {id, 0} otherwise), decoded
 // only where it is used: a load whose value is not read yet costs no wait.
-template <bool A8>
+template <int A8>
 __device__ __forceinline__ int2 load_raw(const Dev &d, int32_t j)
 {
     typedef int v2i __attribute__((ext_vector_type(2)));
+    if (A8 == 2) return make_int2((int32_t)__builtin_nontemporal_load(d.adjc + j), 0);
     if (A8) {
         const v2i e = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(d.adj + j));
         return make_int2(e.x, e.y);
@@ -201,10 +216,12 @@ __device__ __forceinline__ int2 load_raw(const Dev &d, int32_t j)
     return make_int2(__builtin_nontemporal_load(d.nbrs + j), 0);
 }
 
-template <bool A8>
-__device__ __forceinline__ void decode_raw(int2 e, int32_t &u, bool &valid, float &w)
+template <int A8>
+__device__ __forceinline__ void decode_raw(const Dev &d, int2 e, int32_t &u, bool &valid, float &w)
 {
-    if (A8) {
+    if (A8 == 2) {
+        unpack_entry(d, (uint32_t)e.x, u, valid, w);
+    } else if (A8) {
         valid = e.x >= 0;
         u = valid ? e.x : ~e.x;
         w = valid ? __int_as_float(e.y) : -2.0f;
@@ -220,14 +237,14 @@ __device__ __forceinline__ void decode_raw(int2 e, int32_t &u, bool &valid, floa
 // (the caller masks lanes past the row end) -- so the load can be issued
 // unconditionally: a load under a branch makes the compiler wait for every
 // load in flight at the first use after it.
-template <bool A8>
+template <int A8>
 __device__ __forceinline__ int2 first_raw(const Dev &d, const int4 &e, int gl)
 {
     const int32_t j = e.y + gl < e.z ? e.y + gl : (e.z > e.y ? e.z - 1 : 0);
     return load_raw<A8>(d, j);
 }
 
-template <bool A8>
+template <int A8>
 __device__ __forceinline__ void load_entry(const Dev &d, int32_t j, int32_t &u, float &w)
 {
     bool valid;
@@ -350,7 +367,7 @@ __device__ __forceinline__ void first_write(const Dev &d, int32_t v, int32_t vlo
 // Steps 1, 2, ... of a row scan (entries from base on), G lanes per row; the
 // group stops at its first candidate.  Wave-uniform control flow: every lane
 // of the wavefront must call it (done lanes idle).
-template <bool W, bool A8, bool RESPOND, int G>
+template <bool W, int A8, bool RESPOND, int G>
 __device__ __forceinline__ void scan_rest(const Dev &d, int32_t v, int2 rg, int32_t base, bool done, bool &alive,
                                           int32_t &win_u, float &win_w)
 {
@@ -392,7 +409,7 @@ extern __shared__ __align__(16) int4 s_deferred[];
 
 // Second pass over the nd rows deferred to s_def: {v (top bit: alive so
 // far), next entry, row end, loc}, G lanes per row.
-template <bool W, bool A8, bool PUSH, bool RESPOND, int G>
+template <bool W, int A8, bool PUSH, bool RESPOND, int G>
 __device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned long long *resp,
                               uint32_t *s_key = nullptr, int64_t klo = 0, int64_t khi = 0)
 {
@@ -423,7 +440,7 @@ __device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned 
 // return in issue order, so the gathers of the first step are issued before
 // the prefetches: waiting for them then leaves the prefetches (HBM) in flight
 // instead of waiting for those too.
-template <bool W, bool A8, bool PUSH, bool RESPOND, int G>
+template <bool W, int A8, bool PUSH, bool RESPOND, int G>
 __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp,
                                   unsigned long long &nnz)
 {
@@ -468,7 +485,7 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
         int32_t pu;
         bool pok;
         float pw;
-        decode_raw<A8>(praw, pu, pok, pw);
+        decode_raw<A8>(d, praw, pu, pok, pw);
         // (lanes past the row end gather vertex 0's slot: an edgeless graph's
         // row storage holds no ids at all)
         const bool in0 = rg.x + gl < rg.y;
@@ -565,7 +582,7 @@ __device__ void head_scan_range(const Dev &d, int64_t lo, int64_t hi, unsigned l
             const int2 raw[4] = {make_int2(h01.x, h01.y), make_int2(h01.z, h01.w), make_int2(h23.x, h23.y),
                                  make_int2(h23.z, h23.w)};
 #pragma unroll
-            for (int k = 0; k < 4; ++k) decode_raw<true>(raw[k], u[k], ok[k], w[k]);
+            for (int k = 0; k < 4; ++k) decode_raw<1>(d, raw[k], u[k], ok[k], w[k]);
         }
         const int32_t deg0 = rg.y - rg.x;
         const bool valid = vi < hi && sv0 == ST_BLUE && deg0 <= d.hub_deg;
@@ -594,7 +611,7 @@ __device__ void head_scan_range(const Dev &d, int64_t lo, int64_t hi, unsigned l
                     int32_t uj;
                     bool okj;
                     float wj;
-                    load_id<true>(d, j, uj, okj, wj);
+                    load_id<HC ? 2 : 1>(d, j, uj, okj, wj);
                     if (candidate_of<true, false>(0, d.st[uj], 0, okj, alive)) {
                         win_u = uj;
                         win_w = wj;
@@ -606,7 +623,7 @@ __device__ void head_scan_range(const Dev &d, int64_t lo, int64_t hi, unsigned l
         if (valid) nnz += (unsigned long long)deg;
     }
     __syncthreads();
-    deferred_pass<true, true, PUSH, false, G2>(d, s_def, min(s_ndef, DEF_CAP), resp, s_key, lo, hi);
+    deferred_pass<true, HC ? 2 : 1, PUSH, false, G2>(d, s_def, min(s_ndef, DEF_CAP), resp, s_key, lo, hi);
     if (s_key) {
         __syncthreads();
         uint32_t *bk = bucket_keys(d);
@@ -616,7 +633,7 @@ __device__ void head_scan_range(const Dev &d, int64_t lo, int64_t hi, unsigned l
 
 // Rows longer than the hub degree: one block per row, blockDim entries per
 // step, stopping at the step with the first candidate.
-template <bool W, bool A8, bool PUSH, bool RESPOND>
+template <bool W, int A8, bool PUSH, bool RESPOND>
 __device__ void first_scan_hubs(const Dev &d, const int4 *list, int32_t count, int hb, int hub_blocks,
                                 unsigned long long *resp, unsigned long long &nnz)
 {
@@ -1048,7 +1065,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 1024), amdgpu_waves_per
 // Request phase of round r.  Blocks [0, nb) take bucket b of the blue rows,
 // G lanes per row; blocks [nb, nb + hub_blocks) take the hub rows, one block
 // per row.
-template <bool W, bool A8, bool PUSH, int G, bool HEAD = false, bool HC = false>
+template <bool W, int A8, bool PUSH, int G, bool HEAD = false, bool HC = false>
 __global__ void k_request(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
@@ -1130,7 +1147,7 @@ __device__ __forceinline__ uint32_t order_bits(float w)
 
 // RESPOND == false: request scan of blue rows; true: respond scan of red rows.
 // U: row entries each thread has in flight.
-template <bool W, bool A8, bool PUSH, bool RESPOND, int U>
+template <bool W, int A8, bool PUSH, bool RESPOND, int U>
 __device__ void tile_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp)
 {
     extern __shared__ unsigned long long s_dyn[];
@@ -1218,7 +1235,7 @@ __device__ void tile_scan_bucket(const Dev &d, const int4 *list, int32_t count, 
     }
 }
 
-template <bool W, bool A8, bool PUSH, int U>
+template <bool W, int A8, bool PUSH, int U>
 __global__ void k_request_tile(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
@@ -1245,7 +1262,7 @@ __global__ void k_request_tile(Dev d, int r, int hub_blocks)
     }
 }
 
-template <bool W, bool A8, int U>
+template <bool W, int A8, int U>
 __global__ void k_respond_tile(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
@@ -1263,7 +1280,7 @@ __global__ void k_respond_tile(Dev d, int r, int hub_blocks)
 __host__ __device__ constexpr size_t tile_smem_bytes(int T) { return (size_t)T * (8 + 4 * 4) + 17 * 4; }
 
 // Respond phase of round r (PULL): red vertices pick their proposer.
-template <bool W, bool A8, int G>
+template <bool W, int A8, int G>
 __global__ void k_respond(Dev d, int r, int hub_blocks)
 {
     const int parity = r & 1;
@@ -1858,24 +1875,25 @@ constexpr size_t HEAD_SMEM_MAX = 60 * 1024;  // (static LDS of the kernel comes 
 struct RunCfg {
     bool W, A8, PUSH;
     bool HEAD;  // round 1 runs without lists, one lane per row over the head copy
+    bool PK;    // the list rounds' first-candidate scans read the packed 4-byte rows
     int G, U, block, hub_deg, nb, bcap, hub_blocks;
 };
 
-template <bool W, bool A8, bool PUSH, int G>
+template <bool W, int A8, bool PUSH, int G>
 void launch_request_t(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
     hipLaunchKernelGGL((k_request<W, A8, PUSH, G>), dim3(c.nb + c.hub_blocks), dim3(c.block),
                        deferred_smem_bytes(c.block), s, d, r, c.hub_blocks);
 }
 
-template <bool W, bool A8, int G>
+template <bool W, int A8, int G>
 void launch_respond_t(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
     hipLaunchKernelGGL((k_respond<W, A8, G>), dim3(c.nb + c.hub_blocks), dim3(c.block),
                        deferred_smem_bytes(c.block), s, d, r, c.hub_blocks);
 }
 
-template <bool W, bool A8, bool PUSH>
+template <bool W, int A8, bool PUSH>
 void launch_request_g(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
     switch (c.G) {
@@ -1889,7 +1907,7 @@ void launch_request_g(const RunCfg &c, const Dev &d, int r, hipStream_t s)
     }
 }
 
-template <bool W, bool A8>
+template <bool W, int A8>
 void launch_respond_g(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
     switch (c.G) {
@@ -1903,7 +1921,7 @@ void launch_respond_g(const RunCfg &c, const Dev &d, int r, hipStream_t s)
     }
 }
 
-template <bool W, bool A8, bool PUSH>
+template <bool W, int A8, bool PUSH>
 void launch_request_tile(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
     const dim3 grid(c.nb + c.hub_blocks), blk(c.block);
@@ -1933,6 +1951,10 @@ void launch_request(const RunCfg &c, const Dev &d, int r, hipStream_t s)
         }
         return;
     }
+    if (c.PK) {  // (implies weighted PUSH scans with 32-bit keys)
+        launch_request_g<true, 2, true>(c, d, r, s);
+        return;
+    }
     if (c.W) {
         if (c.A8) { if (c.PUSH) launch_request_g<true, true, true>(c, d, r, s); else launch_request_g<true, true, false>(c, d, r, s); }
         else { if (c.PUSH) launch_request_g<true, false, true>(c, d, r, s); else launch_request_g<true, false, false>(c, d, r, s); }
@@ -1942,7 +1964,7 @@ void launch_request(const RunCfg &c, const Dev &d, int r, hipStream_t s)
     }
 }
 
-template <bool W, bool A8>
+template <bool W, int A8>
 void launch_respond_tile(const RunCfg &c, const Dev &d, int r, hipStream_t s)
 {
     const dim3 grid(c.nb + c.hub_blocks), blk(c.block);
@@ -1992,6 +2014,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     // round of weighted PUSH scans (not the tiles); that round then runs
     // without lists.
     c.HEAD = c.A8 && c.PUSH && G != 0 && ((key32 && g->d_headc != nullptr) || g->d_head != nullptr);
+    c.PK = key32 && g->d_adjc != nullptr;
     c.U = o->entries_per_thread > 0 ? o->entries_per_thread : 4;
     if (c.U != 4 && c.U != 8 && c.U != 16)
         return mm::set_error(MM_ERR_INVALID, "mm_match: entries_per_thread must be 4, 8 or 16");
@@ -2030,6 +2053,9 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.head = c.HEAD ? g->d_head : nullptr;
     d.headc = c.HEAD && key32 ? g->d_headc : nullptr;
     if (c.HEAD && !d.head && !d.headc) return mm::set_error(MM_ERR_INVALID, "mm_match: no head copy for round 1");
+    // (the packed rows exist wherever the packed head copy does)
+    d.adjc = key32 ? g->d_adjc : nullptr;
+    if (d.headc && !d.adjc) return mm::set_error(MM_ERR_INVALID, "mm_match: no packed rows beside the packed head copy");
     d.key32 = key32 ? 1 : 0;
     d.listless1 = c.HEAD ? 1 : 0;
     d.no_hubs = g->max_degree <= c.hub_deg ? 1 : 0;

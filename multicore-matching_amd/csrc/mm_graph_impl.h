@@ -44,6 +44,8 @@ struct mm_graph {
                               // 4 per vertex (round-1 scans)
     uint32_t *d_headc = nullptr;  // d_head packed to 4 bytes per entry (invalid << 31 | id << head_pb | position)
     int head_pb = 0;
+    uint32_t *d_adjc = nullptr;   // allocated with d_headc: every entry of d_adjp packed the same way
+                                  // (the list rounds' row scans)
     int32_t *d_match = nullptr;
     int32_t *d_req = nullptr;
     uint8_t *d_st = nullptr;

@@ -886,6 +886,17 @@ __global__ void k_u_adjp(const int2 *rg, const int32_t *order, const int2 *rn, i
     }
 }
 
+// Every entry of the sorted rows with key positions packed in 4 bytes
+// (invalid << 31 | id << pb | position), as the head copy packs its entries.
+__global__ void k_u_pack(const int2 *scan, int64_t nnz, uint32_t *packed, int pb)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x) {
+        const int2 x = scan[i];
+        const uint32_t id = (uint32_t)(x.x >= 0 ? x.x : ~x.x);
+        packed[i] = (x.x < 0 ? 0x80000000u : 0u) | (id << pb) | ((uint32_t)x.y & ((1u << pb) - 1u));
+    }
+}
+
 // The first 4 entries of every scanned row: 8-byte entries, or packed in 4
 // bytes (invalid << 31 | id << pb | position) when pb > 0.
 __global__ void k_u_heads(const int2 *rn, int32_t n, const int2 *scan, int2 *head, uint32_t *headc, int pb)
@@ -1667,6 +1678,13 @@ int32_t mm_upload_finish(mm_graph *g, UploadState *u, bool align_rows, bool head
             if ((rc = mm_graph_alloc(g, (void **)&g->d_headc, sizeof(uint32_t) * 4 * nn, "row heads")) != MM_OK) return rc;
             hipLaunchKernelGGL(k_u_heads, dim3(grid_for(n)), dim3(UB), 0, s, g->d_ranges, n, scan, nullptr, g->d_headc, pb);
             g->head_pb = pb;
+            // (at least one entry: scans of an empty row read entry 0)
+            const int64_t np = std::max<int64_t>(1, (int64_t)nnz_dev);
+            if ((rc = mm_graph_alloc(g, (void **)&g->d_adjc, sizeof(uint32_t) * (size_t)np, "packed rows")) != MM_OK)
+                return rc;
+            if (nnz_dev == 0) UP_TRY(hipMemsetAsync(g->d_adjc, 0, sizeof(uint32_t), s), "packed rows");
+            hipLaunchKernelGGL(k_u_pack, dim3(grid_for((int64_t)nnz_dev)), dim3(UB), 0, s, scan, (int64_t)nnz_dev,
+                               g->d_adjc, pb);
         } else {
             if ((rc = mm_graph_alloc(g, (void **)&g->d_head, sizeof(int2) * 4 * nn, "row heads")) != MM_OK) return rc;
             hipLaunchKernelGGL(k_u_heads, dim3(grid_for(n)), dim3(UB), 0, s, g->d_ranges, n, scan, g->d_head, nullptr, 0);
