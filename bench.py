@@ -167,6 +167,19 @@ def roofline(kb: dict, prof: mmamd.MatchResult, workload: str, tiles: dict) -> t
            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "traffic_tiles": traffic_tiles,
            "traffic_source": source, "launches": launches, "avg_launch_us": round(kms[dom] * 1e3 / launches, 3),
            "bytes_per_launch": kb[dom] / launches}
+    # every kernel class the same way: request and match_select take nearly
+    # equal time on some workloads, so which one is "dominant" can change
+    # between runs
+    by_class = {}
+    for k, ms in kms.items():
+        if ms <= 0.0 or kb.get(k, 0.0) <= 0.0:
+            continue
+        nl = max(1, prof.kernel_launches[k])
+        a = kb[k] / (ms * 1e-3) / 1e9
+        by_class[k] = {"ms": round(ms, 5), "launches": nl, "avg_launch_us": round(ms * 1e3 / nl, 3),
+                       "bytes_per_launch": kb[k] / nl, "achieved": round(a, 1), "frac": round(a / HBM_PEAK_GBS, 4),
+                       "traffic": measured_traffic(workload, k, tiles)[0]}
+    rec["by_class"] = by_class
     return rec, {k: round(v, 5) for k, v in kms.items()}
 
 

@@ -80,6 +80,8 @@ struct Dev {
                           // (graphs with 32-bit keys whose ids and positions fit; null otherwise)
     int32_t head_pb;
     const uint32_t *adjc; // every entry of the sorted rows packed as headc packs them (or null)
+    int32_t *reqs;        // PUSH list rounds: the bucket lists' requests by list slot (bucket b's at
+                          // b * bcap), so the match loads them with the entries; null: by vertex (req)
     int32_t *out;         // result in the caller's order, written as vertices settle
     const int32_t *vbase; // batches: first caller id of each vertex's graph (null: single graph)
     int32_t *req;
@@ -347,10 +349,11 @@ __device__ __forceinline__ unsigned long long key_at(const Dev &d, const unsigne
 template <bool W, bool PUSH, bool RESPOND>
 __device__ __forceinline__ void first_write(const Dev &d, int32_t v, int32_t vloc, int32_t u, float w, bool alive,
                                             unsigned long long *resp, uint32_t *s_key = nullptr, int64_t klo = 0,
-                                            int64_t khi = 0)
+                                            int64_t khi = 0, int32_t *rqa = nullptr)
 {
+    int32_t *const rp = rqa ? rqa : d.req + v;  // (rqa: the request's list slot)
     if (u >= 0) {
-        d.req[v] = u;
+        *rp = u;
         if (!RESPOND && PUSH) {
             if (d.key32) {  // (w holds the position bits of the winning entry)
                 const uint32_t key = ((uint32_t)__float_as_int(w) << d.locbits) | (uint32_t)vloc;
@@ -360,7 +363,7 @@ __device__ __forceinline__ void first_write(const Dev &d, int32_t v, int32_t vlo
                 atomicMax(resp + u, proposal_key<W>(w, vloc));
         }
     } else {
-        d.req[v] = (RESPOND || alive) ? REQ_NONE : REQ_DEAD;
+        *rp = (RESPOND || alive) ? REQ_NONE : REQ_DEAD;
     }
 }
 
@@ -408,10 +411,11 @@ __device__ __forceinline__ void scan_rest(const Dev &d, int32_t v, int2 rg, int3
 extern __shared__ __align__(16) int4 s_deferred[];
 
 // Second pass over the nd rows deferred to s_def: {v (top bit: alive so
-// far), next entry, row end, loc}, G lanes per row.
+// far), next entry, row end, loc}, G lanes per row.  With rq the first
+// field is the row's list slot instead of v: requests go to rq[slot].
 template <bool W, int A8, bool PUSH, bool RESPOND, int G>
 __device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned long long *resp,
-                              uint32_t *s_key = nullptr, int64_t klo = 0, int64_t khi = 0)
+                              uint32_t *s_key = nullptr, int64_t klo = 0, int64_t khi = 0, int32_t *rq = nullptr)
 {
     constexpr int GPW = 64 / G;
     const int lane = threadIdx.x & 63, gl = lane & (G - 1), gw = lane / G;
@@ -427,7 +431,8 @@ __device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned 
         float win_w = 1.0f;
         scan_rest<W, A8, RESPOND, G>(d, v, rg, rg.x, !valid, alive, win_u, win_w);
         alive = (__ballot(alive) & gmask) != 0ull;
-        if (valid && gl == 0) first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp, s_key, klo, khi);
+        if (valid && gl == 0)
+            first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp, s_key, klo, khi, rq ? rq + v : nullptr);
     }
 }
 
@@ -440,9 +445,10 @@ __device__ void deferred_pass(const Dev &d, const int4 *s_def, int nd, unsigned 
 // return in issue order, so the gathers of the first step are issued before
 // the prefetches: waiting for them then leaves the prefetches (HBM) in flight
 // instead of waiting for those too.
+// rq (request scans of PUSH rounds): the requests go to rq[list slot].
 template <bool W, int A8, bool PUSH, bool RESPOND, int G>
 __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count, unsigned long long *resp,
-                                  unsigned long long &nnz)
+                                  unsigned long long &nnz, int32_t *rq = nullptr)
 {
     constexpr int GPW = 64 / G;
     int4 *s_def = s_deferred;
@@ -518,11 +524,14 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
         if (!done && gl == 0) slot = atomicAdd(&s_ndef, 1);
         slot = __shfl(slot, lane & ~(G - 1));
         const bool deferred = slot >= 0 && slot < DEF_CAP;
-        if (deferred && gl == 0) s_def[slot] = make_int4(v | (alive ? INT_MIN : 0), rg.x + G, rg.y, e.w);
+        if (deferred && gl == 0)
+            s_def[slot] = make_int4((rq ? first + gw : v) | (alive ? INT_MIN : 0), rg.x + G, rg.y, e.w);
         scan_rest<W, A8, RESPOND, G>(d, v, rg, rg.x + G, done || deferred, alive, win_u, win_w);
         alive = (__ballot(alive) & gmask) != 0ull;
         if (valid && gl == 0) {
-            if (!deferred) first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp);
+            if (!deferred)
+                first_write<W, PUSH, RESPOND>(d, v, e.w, win_u, win_w, alive, resp, nullptr, 0, 0,
+                                              rq ? rq + first + gw : nullptr);
             nnz += (unsigned long long)(rg.y - rg.x);
         }
         cur = nxe;
@@ -530,7 +539,7 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
         praw = nraw;
     }
     __syncthreads();
-    deferred_pass<W, A8, PUSH, RESPOND, G>(d, s_def, min(s_ndef, DEF_CAP), resp);
+    deferred_pass<W, A8, PUSH, RESPOND, G>(d, s_def, min(s_ndef, DEF_CAP), resp, nullptr, 0, 0, rq);
 }
 
 // Round-1 scans over the head copy, one lane per row, straight over the
@@ -1083,7 +1092,8 @@ __global__ void k_request(Dev d, int r, int hub_blocks)
                                      resp, nnz);
     else
         first_scan_bucket<W, A8, PUSH, false, G>(d, list_ptr(d, parity, L_BLUE_LO) + (size_t)b * d.bcap,
-                                                 ctrb_ptr(d, r, 0)[b], resp, nnz);
+                                                 ctrb_ptr(d, r, 0)[b], resp, nnz,
+                                                 PUSH && d.reqs ? d.reqs + (size_t)b * d.bcap : nullptr);
     if (d.count_nnz) {
         nnz = block_sum(nnz);
         if (threadIdx.x == 0) ctrb64_ptr(d, r, 1)[b] = nnz;
@@ -1425,15 +1435,18 @@ void k_match_select(Dev d, int r, int do_select)
     unsigned long long deg_sum = 0, hub_deg_sum = 0;
     // MU vertices per thread: all loads of a step are issued before any store.
     constexpr int MU = LL ? 2 : 4;
+    // PUSH list rounds: blue vertices' requests by list slot, loaded with the entries
+    const int32_t *rq = PUSH && !LL && d.reqs ? d.reqs + (size_t)b * d.bcap : nullptr;
     for (int base = threadIdx.x & ~63; base < total; base += blockDim.x * MU) {
         int4 e[MU];
-        int32_t v[MU], a[MU], vl[MU], pl[MU];
+        int32_t v[MU], a[MU], vl[MU], pl[MU], ra[MU];
         unsigned long long k[MU];
         int col[MU];
 #pragma unroll
         for (int q = 0; q < MU; ++q) {
             const int t = base + q * blockDim.x + (threadIdx.x & 63);
             col[q] = -1;
+            ra[q] = -1;
             e[q] = make_int4(0, 0, 0, 0);
             if (LL) {
                 if (t < total) {
@@ -1447,6 +1460,7 @@ void k_match_select(Dev d, int r, int do_select)
             } else if (t < total) {
                 col[q] = t < c0 ? ST_BLUE : ST_RED;
                 e[q] = load_ent(t < c0 ? blue + t : red + (t - c0));
+                if (rq && t < c0) ra[q] = rq[t];
             }
             v[q] = e[q].x;
             vl[q] = e[q].w;
@@ -1456,16 +1470,22 @@ void k_match_select(Dev d, int r, int do_select)
             for (int q = 0; q < MU; ++q)
                 if (col[q] >= 0 && e[q].z - e[q].y > d.hub_deg) col[q] = -1;
         }
+        // (with rq a blue vertex's key load goes out with the red ones')
 #pragma unroll
         for (int q = 0; q < MU; ++q) {
             a[q] = -1;
             k[q] = 0ull;
-            if (col[q] >= 0) partner_words<PUSH>(d, v[q], col[q], resp, split, a[q], k[q]);
+            if (rq && col[q] == ST_BLUE) {
+                a[q] = ra[q];
+                if (a[q] >= 0) k[q] = key_at(d, resp, a[q], split);
+            } else if (col[q] >= 0) {
+                partner_words<PUSH>(d, v[q], col[q], resp, split, a[q], k[q]);
+            }
         }
 #pragma unroll
         for (int q = 0; q < MU; ++q) {
             pl[q] = -1;
-            if (col[q] >= 0) partner_words2<PUSH>(d, col[q], resp, split, a[q], k[q], pl[q]);
+            if (col[q] >= 0 && !(rq && col[q] == ST_BLUE)) partner_words2<PUSH>(d, col[q], resp, split, a[q], k[q], pl[q]);
         }
 #pragma unroll
         for (int q = 0; q < MU; ++q) {
@@ -1745,7 +1765,7 @@ void release(mm_graph *g)
     if (!g) return;
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
-    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_adjp,   g->d_head,   g->d_headc,  g->d_match, g->d_req,   g->d_st,
+    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_adjp,   g->d_head,   g->d_headc,  g->d_match, g->d_req,   g->d_st,   g->d_reqs,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_iso,   g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
                     g->d_unperm, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_lcnt, g->d_llist, g->d_params, g->d_vbase, g->d_chk, g->d_chk_keys, g->d_wcsr,
@@ -1819,6 +1839,7 @@ int32_t create_graph(int32_t device, int32_t n, int64_t nnz, const int32_t *rang
     g->nb_max = std::max(1, g->num_cu * 32);
     g->list_cap = (int64_t)n + g->nb_max;
     if ((rc = dev_alloc(g, &g->d_lists, 2 * NLISTS * (size_t)g->list_cap, "work lists")) != MM_OK) return fail(rc);
+    if ((rc = dev_alloc(g, &g->d_reqs, (size_t)g->list_cap, "requests by list slot")) != MM_OK) return fail(rc);
     static_assert(sizeof(*g->d_lists) == 16, "work-list entries are 16 bytes");
     if ((rc = dev_alloc(g, &g->d_ctr, (size_t)2 * ROUND_SLOTS * CTR_STRIDE, "counters")) != MM_OK) return fail(rc);
     if ((rc = dev_alloc(g, &g->d_ctrb, (size_t)ROUND_SLOTS * 4 * g->nb_max, "bucket counters")) != MM_OK)
@@ -2069,6 +2090,9 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     d.out = g->relabelled ? g->d_match_out : g->d_match;
     d.vbase = g->d_vbase;
     d.req = g->d_req;
+    // the first-candidate request scans of PUSH rounds leave the bucket lists'
+    // requests by list slot (the entry-parallel tiles write them by vertex)
+    d.reqs = c.PUSH && c.G != 0 ? g->d_reqs : nullptr;
     d.st = g->d_st;
     d.resp = g->d_resp;
     d.lists = g->d_lists;

@@ -49,6 +49,7 @@ struct mm_graph {
     int32_t *d_match = nullptr;
     int32_t *d_req = nullptr;
     uint8_t *d_st = nullptr;
+    int32_t *d_reqs = nullptr;    // requests of the bucket lists by list slot (PUSH list rounds)
     unsigned long long *d_resp = nullptr;
     int4 *d_lists = nullptr;
     int32_t *d_ctr = nullptr;
