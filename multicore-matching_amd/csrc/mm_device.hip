@@ -10,7 +10,10 @@
 //             of a CSR-order row (grRequest, 286-327).  Round 1 gives each lane
 //             one vertex of its bucket's id range and reads a dense copy of the
 //             rows' first 4 entries; later rounds give G lanes to each entry of
-//             a work list, and rows longer than the hub degree a whole block.
+//             a work list (reading a packed 4-byte copy of the rows when the
+//             graph has one), and rows longer than the hub degree a whole
+//             block.  The list rounds' requests are stored by list slot, so
+//             the match loads them together with the list entries.
 //             PUSH: the proposal is also posted to the red vertex with one
 //             atomic (32-bit min on the proposer's position in the red
 //             vertex's sorted row, or 64-bit max on (weight, -proposer)),
@@ -25,6 +28,10 @@
 //             final value straight into the result in the caller's order, and
 //             the ones still active are recoloured with the next round's seed
 //             (gSelect, 184-218) and appended to the next round's work lists.
+//
+// Vertices without neighbours, when they are at least 1% of a graph, are
+// settled by the round-1 select itself (their whole run follows from their
+// ids and the seeds) and never enter a work list.
 //
 // Only active vertices are touched after round 1: they live in four compacted
 // lists per round (blue/red x row length below or above the hub degree),
