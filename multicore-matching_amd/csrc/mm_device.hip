@@ -46,6 +46,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <climits>
 #include <cstdint>
@@ -2013,6 +2014,18 @@ void launch_respond(const RunCfg &c, const Dev &d, int r, hipStream_t s)
     else { if (c.A8) launch_respond_g<false, true>(c, d, r, s); else launch_respond_g<false, false>(c, d, r, s); }
 }
 
+// Waits for the stream: polls for up to 2 ms (a step of a large graph ends
+// within that, and polling wakes the caller about 2 us sooner than a blocking
+// wait), then blocks.
+hipError_t wait_stream(hipStream_t s)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    hipError_t q;
+    while ((q = hipStreamQuery(s)) == hipErrorNotReady)
+        if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) return hipStreamSynchronize(s);
+    return q;
+}
+
 int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector<int32_t> *graph_rounds)
 {
     if (!o) return mm::set_error(MM_ERR_INVALID, "mm_match: opts is NULL");
@@ -2257,7 +2270,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
                                           sizeof(int32_t) * (size_t)(k_hi - r + 1) * CTR_STRIDE,
                                           hipMemcpyDeviceToHost, g->stream),
                            "Unable to read round counters");
-                MM_HIP_TRY(hipStreamSynchronize(g->stream), "A HIP error occurred during the matching process");
+                MM_HIP_TRY(wait_stream(g->stream), "A HIP error occurred during the matching process");
                 auto active = [&](int k) {
                     const int32_t *q = g->h_ctr + (size_t)k * CTR_STRIDE;
                     return (int64_t)q[0] + q[1] + q[2] + q[3];
@@ -2305,7 +2318,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
                        "Unable to read batch rounds");
         }
     }
-    MM_HIP_TRY(hipStreamSynchronize(g->stream), "A HIP error occurred during the matching process");
+    MM_HIP_TRY(wait_stream(g->stream), "A HIP error occurred during the matching process");
     MM_HIP_TRY(hipGetLastError(), "A HIP error occurred during the matching process");
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));

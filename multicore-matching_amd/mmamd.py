@@ -56,7 +56,7 @@ class _Opts(ctypes.Structure):
         ("maximal", ctypes.c_int32),
         ("max_rounds", ctypes.c_int32),
         ("barrier", ctypes.c_uint32),
-        ("seeds", ctypes.POINTER(ctypes.c_uint32)),
+        ("seeds", ctypes.c_void_p),  # const uint32_t *
         ("n_seeds", ctypes.c_int32),
         ("block_size", ctypes.c_int32),
         ("lanes_per_vertex", ctypes.c_int32),
@@ -376,23 +376,21 @@ class MatchResult:
     kernel_launches: dict = field(default_factory=dict)
 
 
+def _address(a: np.ndarray) -> int:
+    """Data address of a contiguous array (the buffer route is several times
+    faster than numpy's ctypes attribute, which matters at one call per
+    sub-millisecond matching; read-only arrays take the attribute)."""
+    try:
+        return ctypes.addressof(ctypes.c_char.from_buffer(a))
+    except (TypeError, ValueError):
+        return a.ctypes.data
+
+
 def _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond, profile,
                graph_mode=0, unroll=0):
     seeds = np.ascontiguousarray(seeds, dtype=np.uint32)
-    o = _Opts()
-    o.kind = kind
-    o.maximal = int(maximal)
-    o.max_rounds = max_rounds
-    o.barrier = barrier
-    o.seeds = seeds.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
-    o.n_seeds = len(seeds)
-    o.block_size = block_size
-    o.lanes_per_vertex = lanes
-    o.hub_degree = hub_degree
-    o.respond = respond
-    o.profile = int(profile)
-    o.graph_mode = int(graph_mode)
-    o.entries_per_thread = int(unroll)
+    o = _Opts(kind, int(maximal), max_rounds, barrier, _address(seeds) if len(seeds) else None, len(seeds),
+              block_size, lanes, hub_degree, int(unroll), respond, int(profile), int(graph_mode))
     return o, seeds
 
 
