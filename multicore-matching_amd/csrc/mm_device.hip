@@ -2256,7 +2256,8 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
         return MM_OK;
     };
 
-    MM_HIP_TRY(hipEventRecord(g->ev_t1, g->stream), "hipEventRecord");
+    // (the events time the rounds for stats->match_ms only)
+    if (stats) MM_HIP_TRY(hipEventRecord(g->ev_t1, g->stream), "hipEventRecord");
     int rounds = -1;
     int r = 1;
     if (g->n > 0) {
@@ -2290,7 +2291,7 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
     // The result is already in the caller's numbering; the internal-order
     // array is rebuilt from it only if a device check asks for it.
     g->internal_stale = g->relabelled;
-    MM_HIP_TRY(hipEventRecord(g->ev_t2, g->stream), "hipEventRecord");
+    if (stats) MM_HIP_TRY(hipEventRecord(g->ev_t2, g->stream), "hipEventRecord");
     // Totals of every executed round (a maximal run's last, empty round has none).
     const int r_stats = g->n > 0 ? std::min(rounds, R_run) : 0;
     if (stats && r_stats > 0) {
