@@ -1414,8 +1414,9 @@ __device__ __forceinline__ void partner_words2(const Dev &d, int colour, const u
 // TIGHT: held to 64 VGPRs (8 wavefronts per SIMD, a few spilled values), so
 // two blocks of 1024 threads share a CU and all the buckets' blocks of a
 // 2^22-vertex graph run at once: its list rounds 2-5 took 30/20/14/9 instead
-// of 35/23/17/12 us.  Batches of small graphs keep 72 registers (4.98 instead
-// of 4.89 ms per batch of 4096 when held).  Round 1's pass over every id takes
+// of 35/23/17/12 us.  Batches of small graphs keep 72 registers in rounds 2-4
+// (held there too, a batch of 4096 took 4.62 instead of 4.50 ms) and are held
+// from round 5 on, when few vertices are left.  Round 1's pass over every id takes
 // 2 vertices per thread instead of 4 (56 registers, no spills) and also runs
 // 8 wavefronts per SIMD (61 instead of 64 us).
 template <bool PUSH, bool LL = false, bool TIGHT = false>
@@ -2208,7 +2209,11 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
             MM_LAUNCH(K_REQUEST, launch_request(cr, d, rr, g->stream));
             if (!c.PUSH) MM_LAUNCH(K_RESPOND, launch_respond(cr, d, rr, g->stream));
             const bool ll = rr == 1 && d.listless1;
-            const bool tight = g->n_graphs == 0;  // (a single graph)
+            // the 64-register match build: a single graph's list rounds, and a
+            // batch's from round 5, when its buckets hold few vertices and two
+            // blocks per CU halve the launch's block waves (batch of 4096:
+            // 4.47 -> 4.42 ms; from round 3 on it is slower)
+            const bool tight = g->n_graphs == 0 || rr >= 5;
             if (c.PUSH && ll) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
             else if (c.PUSH && tight) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
             else if (c.PUSH) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
