@@ -2213,13 +2213,22 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
             // batch's from round 5, when its buckets hold few vertices and two
             // blocks per CU halve the launch's block waves (batch of 4096:
             // 4.47 -> 4.42 ms; from round 3 on it is slower)
-            const bool tight = g->n_graphs == 0 || rr >= 5;
+            bool tight = g->n_graphs == 0 || rr >= 5;
+            // From round 3 on a single graph's buckets hold a few hundred
+            // vertices: 512-thread blocks of the unbounded build (no spills,
+            // three blocks per CU) beat 1024-thread blocks of the held one
+            // (RGG-22: 3 us per step; 256 threads or from round 2 are slower).
+            int mblock = block;
+            if (g->n_graphs == 0 && rr >= 3 && block > 512) {
+                mblock = 512;
+                tight = false;
+            }
             if (c.PUSH && ll) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
-            else if (c.PUSH && tight) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
-            else if (c.PUSH) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
+            else if (c.PUSH && tight) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false, true>), dim3(c.nb), dim3(mblock), 0, g->stream, d, rr, do_select));
+            else if (c.PUSH) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<true, false>), dim3(c.nb), dim3(mblock), 0, g->stream, d, rr, do_select));
             else if (ll) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
-            else if (tight) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, false, true>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
-            else MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, false>), dim3(c.nb), dim3(block), 0, g->stream, d, rr, do_select));
+            else if (tight) MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, false, true>), dim3(c.nb), dim3(mblock), 0, g->stream, d, rr, do_select));
+            else MM_LAUNCH(K_MATCH, hipLaunchKernelGGL((k_match_select<false, false>), dim3(c.nb), dim3(mblock), 0, g->stream, d, rr, do_select));
         }
         if (maximal) return enq_reduce(r0, std::min(r1 + 1, R_run));
         return MM_OK;
