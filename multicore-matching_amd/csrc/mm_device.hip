@@ -1621,6 +1621,30 @@ __global__ __attribute__((amdgpu_flat_work_group_size(1, 1024))) void k_reduce_c
     }
 }
 
+// Writes the state array back in the caller's vertex numbering: coalesced
+// writes, reads gathered from the (L2/MALL-resident) state array.
+// Offsets of the eight groups of (output position, internal id) pairs.
+struct UnpermOff {
+    int32_t off[9];
+};
+
+// out[o] = match[v] for every pair {o, v}.  The pairs are grouped by the
+// eighth of the internal ids that v falls in (ordered by o within a group), and
+// workgroup i works on group i % 8: workgroups are dealt to the 8 XCDs round
+// robin, so each XCD gathers from one 1/8 slice of match[], which stays in its
+// own L2, instead of every XCD missing on random lines of the whole array.
+__global__ void k_unpermute(const int2 *__restrict__ pairs, UnpermOff g, const int32_t *__restrict__ match,
+                            int32_t *__restrict__ out)
+{
+    const int x = blockIdx.x & 7;
+    const int64_t step = (int64_t)(gridDim.x >> 3) * blockDim.x;
+    for (int64_t k = g.off[x] + (int64_t)(blockIdx.x >> 3) * blockDim.x + threadIdx.x; k < g.off[x + 1]; k += step) {
+        typedef int v2i __attribute__((ext_vector_type(2)));
+        const v2i p = __builtin_nontemporal_load(reinterpret_cast<const v2i *>(pairs + k));
+        out[p.x] = match[p.y];
+    }
+}
+
 // Rounds each batched graph would execute alone: one plus the last round in
 // which one of its vertices was still active (src/matchgpu.cu:604 loop).
 // Three launches: clear the per-graph maxima, fold the vertices' last rounds
@@ -2471,6 +2495,18 @@ int predicted_rounds(int64_t before, int64_t now)
     const double shrink = std::log((double)before / (double)now);
     const double need = std::log((double)now + 1.0) / std::max(shrink, 1e-3) + 2.0;
     return (int)std::min(32.0, std::max(2.0, std::ceil(need)));
+}
+
+void mm_graph_enqueue_unpermute(mm_graph *g)
+{
+    if (g->relabelled && g->n > 0) {
+        UnpermOff off;
+        std::memcpy(off.off, g->unperm_off, sizeof(off.off));
+        const int per_xcd = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)g->n / 8 + 255) / 256,
+                                                                          (int64_t)std::max(1, g->num_cu / 8) * 4));
+        hipLaunchKernelGGL(k_unpermute, dim3(8 * per_xcd), dim3(256), 0, g->stream, (const int2 *)g->d_unperm, off,
+                           (const int32_t *)g->d_match, g->d_match_out);
+    }
 }
 
 extern "C" {

@@ -72,6 +72,7 @@ struct mm_graph {
     int32_t n_graphs = 0;
     int32_t *d_loc = nullptr;        // graph-local original id per internal vertex
     int2 *d_unperm = nullptr;        // {original position, internal id} pairs, grouped by eighth of internal ids
+    int32_t unperm_off[9] = {0};     // group offsets in d_unperm
     int32_t *d_match_out = nullptr;  // result in the caller's numbering, when relabelled
     int32_t *d_vbase = nullptr;      // batches: first global id of each internal vertex's graph
     void *d_chk = nullptr;           // scratch of the device result check (mm_check.hip lays it out)
@@ -133,6 +134,7 @@ int32_t mm_upload_finish(mm_graph *g, UploadState *u, bool align_rows, bool head
 void mm_upload_free(UploadState *u);
 // Enqueues the copy of the internal match array into the caller's vertex order
 // (relabelled graphs only; a no-op otherwise).
+void mm_graph_enqueue_unpermute(mm_graph *g);
 // The static scratch of the no-allocation check on `device` (the current
 // device): its device address and size (mm_scratch.hip).
 int32_t mm_noalloc_scratch(int device, char **out, size_t *bytes);

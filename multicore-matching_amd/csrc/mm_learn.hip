@@ -153,8 +153,7 @@ struct LDev {
     const int32_t *nbrs;
     const int2 *adj;
     const int32_t *loc;
-    int32_t *out;          // the result in the caller's order, at vbase[v] + loc[v]
-    const int32_t *vbase;  // batches: each vertex's graph's first caller id (null: 0)
+    int32_t *match;
     int32_t *req;
     uint8_t *col;
     uint8_t *st;
@@ -207,10 +206,12 @@ __device__ __forceinline__ int block_sum(int x)
     return s;
 }
 
-__global__ void k_l_start(uint8_t *st, int64_t n)
+__global__ void k_l_start(int32_t *match, uint8_t *st, int64_t n)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        match[i] = 0;
         st[i] = 0;
+    }
 }
 
 __global__ void k_l_fill(int32_t *p, int64_t count, int32_t value)
@@ -358,9 +359,8 @@ __global__ __launch_bounds__(LTILE) void k_l_select(LDev d, int r, uint32_t seed
         if (base + t < limit) c0 = from_list ? prev[base + t] : (int32_t)(base + t);
         if (base + LTILE + t < limit) c1 = from_list ? prev[base + LTILE + t] : (int32_t)(base + LTILE + t);
         int A0, A1;
-        // (a vertex is active while its state is a colour: st < 2)
-        const bool act0 = c0 >= 0 && d.st[c0] < 2;
-        const bool act1 = c1 >= 0 && d.st[c1] < 2;
+        const bool act0 = c0 >= 0 && d.match[c0] < 2;
+        const bool act1 = c1 >= 0 && d.match[c1] < 2;
         const int pos0 = block_prefix(act0, A0);
         if (act0) s_idx[pos0] = c0;
         const int pos1 = block_prefix(act1, A1);
@@ -617,10 +617,7 @@ __global__ __launch_bounds__(LTILE) void k_l_respond(LDev d)
 
 // Match: block b's active list, one thread per vertex.
 template <bool PUSH>
-// Settled vertices store their final value straight into the caller-order
-// result, as the plain rounds do; in the run's last round (last) every vertex
-// still active stores its colour.
-__global__ __launch_bounds__(LTILE) void k_l_match(LDev d, int r, uint32_t seed, int last)
+__global__ __launch_bounds__(LTILE) void k_l_match(LDev d, int r, uint32_t seed)
 {
     const int32_t cnt = d.lcnt[blockIdx.x];
     const int32_t *list = d.lact + (size_t)blockIdx.x * d.seg;
@@ -658,7 +655,7 @@ __global__ __launch_bounds__(LTILE) void k_l_match(LDev d, int r, uint32_t seed,
             } else {
                 nm = c;
             }
-            if (nm >= 2 || last) d.out[(d.vbase ? d.vbase[v] : 0) + (int32_t)vl] = nm;
+            d.match[v] = nm;
             d.st[v] = (uint8_t)(nm >= 4 ? 3 : nm);
             if (nm >= 2) d.col[v] = LC_DONE;
         }
@@ -987,8 +984,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
     d.nbrs = g->d_nbrs;
     d.adj = g->d_adj;
     d.loc = g->d_loc;
-    d.out = g->relabelled ? g->d_match_out : g->d_match;
-    d.vbase = g->d_vbase;
+    d.match = g->d_match;
     d.req = g->d_req;
     d.col = g->d_lcol;
     d.st = g->d_st;
@@ -1035,7 +1031,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
     ML_TRY(hipEventRecord(g->ev_t1, st), "hipEventRecord");
     hipLaunchKernelGGL(k_l_fill, dim3(1), dim3(256), 0, st, d.ctr, (int64_t)LSLOTS * LCTR, 0);
     hipLaunchKernelGGL(k_l_fill, dim3(1), dim3(64), 0, st, (int32_t *)d.scount, (int64_t)4, 0);
-    if (g->n > 0) hipLaunchKernelGGL(k_l_start, dim3(grid), dim3(block), 0, st, d.st, (int64_t)g->n);
+    if (g->n > 0) hipLaunchKernelGGL(k_l_start, dim3(grid), dim3(block), 0, st, d.match, d.st, (int64_t)g->n);
     const int R_run = maximal ? std::min(R_max, o->n_seeds) : R_max;
     // Maximal runs read the counters back every 4 rounds: an empty learned round
     // still sweeps the select (about as costly as a read-back), so the plain
@@ -1056,7 +1052,7 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         hipLaunchKernelGGL((k_l_select<A8_, PUSH_, SEL_G>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed, P); \
         hipLaunchKernelGGL((k_l_request<A8_, PUSH_, SCAN_G>), dim3(sgrid), dim3(LTILE), 0, st, d);            \
         if (!PUSH_) hipLaunchKernelGGL((k_l_respond<A8_, SCAN_G>), dim3(sgrid), dim3(LTILE), 0, st, d);        \
-        hipLaunchKernelGGL((k_l_match<PUSH_>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed, rr == R_run ? 1 : 0); \
+        hipLaunchKernelGGL((k_l_match<PUSH_>), dim3(sgrid), dim3(LTILE), 0, st, d, rr, seed);                 \
     } while (0)
             if (A8 && push) ML_ROUND(true, true);
             else if (A8) ML_ROUND(true, false);
@@ -1085,9 +1081,8 @@ int32_t run_learned(mm_graph *g, const mm_opts *o, mm_learn_opts *lo, mm_stats *
         return mm::set_error(MM_ERR_SEEDS, "mm_match_learned: ran out of seeds before the matching converged");
     if (g->n == 0) rounds = maximal ? 1 : R_max;
     if (rounds < 0) rounds = R_max;
-    // The result is already in the caller's numbering; the internal-order
-    // array is rebuilt from it only if a device check asks for it.
-    g->internal_stale = g->relabelled;
+    mm_graph_enqueue_unpermute(g);
+    g->internal_stale = false;
     ML_TRY(hipEventRecord(g->ev_t2, st), "hipEventRecord");
     ML_TRY(hipMemcpyAsync(g->h_lctr, d.ctr, sizeof(int32_t) * LSLOTS * LCTR, hipMemcpyDeviceToHost, st),
            "Unable to read round counters");

@@ -1709,6 +1709,11 @@ int32_t mm_upload_finish(mm_graph *g, UploadState *u, bool align_rows, bool head
             return rc;
         hipLaunchKernelGGL(k_u_pairs_out, dim3(grid_for(n)), dim3(UB), 0, s, vb, inv, n, g->d_unperm);
         UP_TRY(hipGetLastError(), "k_u_pairs_out");
+        int32_t h_cnt[8];
+        UP_TRY(hipMemcpyAsync(h_cnt, cnt, sizeof(h_cnt), hipMemcpyDeviceToHost, s), "vertex order");
+        UP_TRY(hipStreamSynchronize(s), "vertex order");
+        g->unperm_off[0] = 0;
+        for (int x = 0; x < 8; ++x) g->unperm_off[x + 1] = g->unperm_off[x] + h_cnt[x];
     }
     UP_TRY(hipStreamSynchronize(s), "Unable to prepare the graph on the device");
     mm_upload_mark("heads and caller pairs", s);
