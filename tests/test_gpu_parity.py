@@ -216,6 +216,39 @@ def test_proposal_key_widths():
         g.close()
 
 
+def test_weights_that_never_win_through_packed_rows():
+    """Weights of -1 and below can never win a request (the reference starts
+    its scan from -1.0 with a strict '>', src/matchgpu.cu:360-409); the sorted
+    rows put them last with the id marked.  A graph with 32-bit keys and a packed
+    head copy also scans packed 4-byte rows in the list rounds, where the mark is
+    the entry's top bit: symmetric weights of -1 and -1.5 on a third of the
+    edges of an RGG must give the oracle's result in every round and tile."""
+    base = mmamd.gen_rgg(1 << 12, seed=31, weighted=True)
+    r = base.ranges
+    rows = np.repeat(np.arange(base.n), r[:, 1] - r[:, 0])
+    keep = rows < base.neighbours
+    e = np.stack([rows[keep], base.neighbours[keep]], axis=1).astype(np.int64)
+    w = base.weights[keep].copy()
+    h = (e[:, 0] * 1315423911 + e[:, 1] * 2654435761) % 9
+    w[h < 2] = np.float32(-1.0)
+    w[(h >= 2) & (h < 3)] = np.float32(-1.5)
+    csr = CSR.from_edges(base.n, e, w)
+    for keep_order in (False, True):
+        g = Graph(csr, keep_order=keep_order)
+        info = g.info()
+        assert info["push_ok"] and info["key_bits"] == 32
+        for maximal, rounds in ((True, 0), (False, 3)):
+            for lanes, block, hub in ((8, 1024, 256), (2, 256, 256), (16, 64, 8)):
+                say("never-win weights keep", keep_order, "maximal", maximal, "tiles", lanes, block, hub)
+                exp, R, act, mat = oracle.match(csr, True, maximal, SEEDS, max_rounds=rounds, nthreads=0)
+                res = g.match(kind=MM_WEIGHTED, maximal=maximal, seeds=SEEDS, max_rounds=rounds, lanes=lanes,
+                              block_size=block, hub_degree=hub, respond=MM_RESPOND_PUSH)
+                assert res.rounds == R
+                np.testing.assert_array_equal(res.match, exp)
+                np.testing.assert_array_equal(res.active[:R], act)
+        g.close()
+
+
 def test_round_one_bucket_keys_with_and_without_room():
     """Round 1 combines the proposals between rows of one bucket in LDS when
     the bucket's 32-bit keys fit beside the deferred rows, and posts every
