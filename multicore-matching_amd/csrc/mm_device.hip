@@ -2217,7 +2217,8 @@ int32_t run_matching(mm_graph *g, const mm_opts *o, mm_stats *stats, std::vector
             // From round 3 on a single graph's buckets hold a few hundred
             // vertices: 512-thread blocks of the unbounded build (no spills,
             // three blocks per CU) beat 1024-thread blocks of the held one
-            // (RGG-22: 3 us per step; 256 threads or from round 2 are slower).
+            // (RGG-22: 0.315 -> 0.308 ms per step; 256 threads, or from round 2
+            // on, are slower).
             int mblock = block;
             if (g->n_graphs == 0 && rr >= 3 && block > 512) {
                 mblock = 512;
