@@ -70,6 +70,25 @@ def test_isolated_vertices_settled_by_the_select_match_the_oracle(keep):
         g.close()
 
 
+def test_graphs_without_any_edge():
+    """Every vertex without neighbours: the select settles all of them, the
+    first round's request and match scan an empty id range, and a maximal run
+    ends when the last red vertex has been blue once."""
+    for n in (1, 100, 70000):
+        for weighted in (True, False):
+            w = np.ones(0, np.float32) if weighted else None
+            csr = mmamd.CSR.from_offsets(np.zeros(n + 1, np.int64), np.zeros(0, np.int32), w)
+            for keep in (False, True):
+                g = Graph(csr, keep_order=keep)
+                modes = [MM_RESPOND_PULL] + ([MM_RESPOND_PUSH] if g.info()["push_ok"] else [])
+                for respond in modes:
+                    for maximal, rounds in ((True, 0), (False, 1), (False, 4)):
+                        say("edgeless n", n, "weighted", weighted, "keep", keep, "respond", respond,
+                            "maximal", maximal, "rounds", rounds)
+                        compare(csr, g, weighted, maximal, rounds, respond=respond)
+                g.close()
+
+
 def test_graphs_without_many_isolated_vertices_keep_them_in_the_rounds():
     csr = mmamd.gen_rgg(1 << 14, seed=3, weighted=True)
     assert isolated_fraction(csr) < 0.01
