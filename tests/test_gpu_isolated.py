@@ -89,6 +89,25 @@ def test_graphs_without_any_edge():
                 g.close()
 
 
+def test_threshold_of_settling_vertices_without_neighbours():
+    """The select settles vertices without neighbours when they are at least
+    1% of the graph: exactly 1% takes that path, one vertex fewer does not, and
+    both give the oracle's result."""
+    base = mmamd.gen_rgg(9900, seed=41, weighted=True, radius_scale=1.2)
+    r = base.ranges
+    assert np.all(r[:, 1] > r[:, 0])
+    for extra, settled in ((100, 1), (99, 0)):
+        n = base.n + extra
+        offsets = np.concatenate([r[:, 0], [r[-1, 1]] * (extra + 1)]).astype(np.int64)
+        csr = mmamd.CSR.from_offsets(offsets, base.neighbours, base.weights)
+        g = Graph(csr)
+        assert g.info()["iso_settled"] == (1 if (n - base.n) * 100 >= n else 0) == settled, (extra, n)
+        for maximal, rounds in ((True, 0), (False, 3)):
+            say("threshold extra", extra, "maximal", maximal)
+            compare(csr, g, True, maximal, rounds)
+        g.close()
+
+
 def test_graphs_without_many_isolated_vertices_keep_them_in_the_rounds():
     csr = mmamd.gen_rgg(1 << 14, seed=3, weighted=True)
     assert isolated_fraction(csr) < 0.01

@@ -249,6 +249,35 @@ def test_weights_that_never_win_through_packed_rows():
         g.close()
 
 
+def test_packed_entries_at_their_widest():
+    """Packed head and row entries hold an invalid bit, the id and the key
+    position in 32 bits; a 2^16-vertex graph with a 20000-entry row needs
+    16 + 15 of the 31 bits, the most the packing allows."""
+    rng = np.random.default_rng(43)
+    m = 1 << 16
+    ring = np.stack([np.arange(m), (np.arange(m) + 1) % m], axis=1)
+    star = np.stack([np.full(20000, 7, np.int64), rng.choice(np.arange(m)[np.arange(m) != 7], 20000, replace=False)],
+                    axis=1)
+    chords = rng.integers(0, m, size=(m, 2))
+    e = np.concatenate([ring, star, chords])
+    e = np.unique(np.sort(e[e[:, 0] != e[:, 1]], axis=1), axis=0)
+    w = (rng.integers(0, 1 << 12, size=len(e)) + 1).astype(np.float32) / np.float32(1 << 12)
+    csr = CSR.from_edges(m, e, w)
+    # the packed head copy (16 bytes per vertex) comes with the packed rows (4
+    # bytes per entry): dropping the head copy frees both; an unpacked head copy
+    # would free only its 32 bytes per vertex
+    bare = Graph(csr, heads=False)
+    g = Graph(csr)
+    info = g.info()
+    assert info["push_ok"] and info["key_bits"] == 32
+    assert info["device_bytes"] - bare.info()["device_bytes"] >= 16 * m + 4 * csr.nnz
+    bare.close()
+    for lanes, hub in ((8, 256), (4, 32768), (16, 64)):
+        say("widest packed entries, tiles", lanes, hub)
+        check_against_oracle(csr, g, True, True, lanes=lanes, hub_degree=hub)
+    g.close()
+
+
 def test_round_one_bucket_keys_with_and_without_room():
     """Round 1 combines the proposals between rows of one bucket in LDS when
     the bucket's 32-bit keys fit beside the deferred rows, and posts every
