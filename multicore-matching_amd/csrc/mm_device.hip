@@ -1774,7 +1774,7 @@ void release(mm_graph *g)
     if (!g) return;
     hipSetDevice(g->device);
     if (g->stream) hipStreamSynchronize(g->stream);
-    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_adjp,   g->d_head,   g->d_headc,  g->d_match, g->d_req,   g->d_st,   g->d_reqs,
+    void *ptrs[] = {g->d_ranges, g->d_nbrs,  g->d_adj,    g->d_adjp,   g->d_head,   g->d_headc,  g->d_adjc,  g->d_match, g->d_req,   g->d_st,   g->d_reqs,
                     g->d_resp,   g->d_lists, g->d_ctr,    g->d_ctrb,  g->d_ctr64, g->d_ctrb64,
                     g->d_hub64,  g->d_iso,   g->d_seeds, g->d_loc,    g->d_voff,  g->d_vround, g->d_grounds,
                     g->d_unperm, g->d_match_out, g->d_lcol, g->d_lreqw, g->d_lfeat, g->d_lctr, g->d_lpart, g->d_lcnt, g->d_llist, g->d_params, g->d_vbase, g->d_chk, g->d_chk_keys, g->d_wcsr,
