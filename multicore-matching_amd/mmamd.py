@@ -438,12 +438,22 @@ class Graph:
         stats=False skips the counter read-back and returns None."""
         if seeds is None:
             seeds = reference_seeds(MM_MAX_ROUNDS)
-        o, keep = _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond,
-                             profile, graph_mode, unroll)
+        # A repeated call with the same arguments and the same uint32 seeds
+        # array reuses its option block (the library reads the seeds' current
+        # values through it; the reference held here keeps numpy from resizing
+        # the array under it).
+        key = (kind, maximal, max_rounds, barrier, block_size, lanes, hub_degree, respond, profile, graph_mode,
+               unroll)
+        c = getattr(self, "_opts_cache", None)
+        if c is not None and c[0] == key and c[2] is seeds:
+            o = c[1]
+        else:
+            o, keep = _make_opts(kind, maximal, seeds, max_rounds, barrier, block_size, lanes, hub_degree, respond,
+                                 profile, graph_mode, unroll)
+            self._opts_cache = (key, o, keep) if keep is seeds else None
         st = _Stats() if stats else None
         out = np.empty(self.n, np.int32) if fetch else None
         _check(lib().mm_match(self._h, ctypes.byref(o), _ptr(out), ctypes.byref(st) if stats else None))
-        del keep
         return _result(st, out) if stats else None
 
     def match_learned(self, params=None, kind: int = MM_WEIGHTED, maximal: bool = True, seeds=None,

@@ -278,6 +278,21 @@ def test_packed_entries_at_their_widest():
     g.close()
 
 
+def test_repeated_calls_follow_the_seeds_and_arguments():
+    """Repeated calls with the same arguments reuse their option block: seeds
+    changed in place between calls, and a changed argument, must still give the
+    oracle's result for the values of that call."""
+    csr = mmamd.gen_rgg(1 << 14, seed=61, weighted=True)
+    g = Graph(csr)
+    s = reference_seeds(255).copy()
+    for rounds in (3, 3, 4):
+        res = g.match(kind=MM_WEIGHTED, maximal=False, max_rounds=rounds, seeds=s)
+        exp = oracle.match(csr, True, False, s.copy(), max_rounds=rounds, nthreads=0)[0]
+        assert res.match.tolist() == exp.tolist(), rounds
+        s[:] = np.roll(s, 1)  # the same array, new values
+    g.close()
+
+
 def test_round_one_bucket_keys_with_and_without_room():
     """Round 1 combines the proposals between rows of one bucket in LDS when
     the bucket's 32-bit keys fit beside the deferred rows, and posts every
