@@ -508,8 +508,11 @@ __device__ void first_scan_bucket(const Dev &d, const int4 *list, int32_t count,
         const int32_t q0 = RESPOND ? d.req[g0] : 0;
         // keep the scheduler from hoisting the prefetches above the gathers
         __builtin_amdgcn_sched_barrier(0);
-        const int4 nn = ent_at(first + 2 * step);
+        // (the next entry's words are shuffled out before the load two steps
+        // ahead is issued: the load can then take their register, and the
+        // loop-carried copy that would wait for it goes away)
         const int4 nxe = ent_of(nxt);
+        const int4 nn = ent_at(first + 2 * step);
         const int2 nraw = first_raw<A8>(d, nxe, gl);
         // step 0
         int32_t win_u = -1;
