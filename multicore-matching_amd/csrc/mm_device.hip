@@ -1540,8 +1540,11 @@ void k_match_select(Dev d, int r, int do_select)
         wave_append(cls == L_BLUE_HI, ent, list_ptr(d, nparity, L_BLUE_HI), d.ctr + (r + 1) * CTR_STRIDE + L_BLUE_HI);
         wave_append(cls == L_RED_HI, ent, list_ptr(d, nparity, L_RED_HI), d.ctr + (r + 1) * CTR_STRIDE + L_RED_HI);
     }
-    n_matched = block_sum(n_matched);
-    n_dead = block_sum(n_dead);
+    // the matched and dead counts share one reduction (low and high halves)
+    const unsigned long long md = block_sum((unsigned long long)(uint32_t)n_matched |
+                                            ((unsigned long long)(uint32_t)n_dead << 32));
+    n_matched = (int32_t)(uint32_t)md;
+    n_dead = (int32_t)(md >> 32);
     if (d.count_nnz) {
         deg_sum = block_sum(deg_sum);
         hub_deg_sum = block_sum(hub_deg_sum);
