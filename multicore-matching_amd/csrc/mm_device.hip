@@ -1553,7 +1553,8 @@ void k_match_select(Dev d, int r, int do_select)
             if (hub_deg_sum) atomicAdd(d.hub64 + (size_t)(r + 1) * 2 + 0, hub_deg_sum);
         }
     }
-    __syncthreads();
+    // (every wavefront made its appends to s_cnt before entering block_sum,
+    // whose barriers order them before thread 0's reads)
     if (threadIdx.x == 0) {
         ctrb_ptr(d, r, 2)[b] = n_matched;
         ctrb_ptr(d, r, 3)[b] = n_dead;
